@@ -1,0 +1,342 @@
+"""ctypes binding of ``libfddm_hip.so`` (declared in include/fddm_hip.h).
+
+Plays the role of the reference's pybind11 module ``_core_cuda``
+(src/python/module_cuda.cc, src/python/bindings/ddm_cuda_bindings.h:24-56):
+device utilities plus the fft/diff entry points.  Every failing call raises
+``RuntimeError`` with the library's message; the library is required -- there
+is no fallback path.  ctypes releases the GIL for the duration of each call.
+"""
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from ._config import HIP_LIBRARY_PATH
+
+DTYPE_CODES = {
+    np.dtype(np.uint8): 0,
+    np.dtype(np.int16): 1,
+    np.dtype(np.uint16): 2,
+    np.dtype(np.int32): 3,
+    np.dtype(np.uint32): 4,
+    np.dtype(np.int64): 5,
+    np.dtype(np.uint64): 6,
+    np.dtype(np.float32): 7,
+    np.dtype(np.float64): 8,
+}
+PRECISIONS = {"fp64": 0, "fp32": 1, "fp32c": 2}
+OUT_CODES = {np.dtype(np.float64): 0, np.dtype(np.float32): 1}
+MODE_DIFF, MODE_FFT = 0, 1
+_ERRLEN = 1024
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    try:
+        L = ctypes.CDLL(HIP_LIBRARY_PATH)
+    except OSError as exc:
+        raise RuntimeError(
+            f"fastddm_amd HIP library not loadable ({HIP_LIBRARY_PATH}): {exc}. "
+            "Build it with `python -m fastddm_amd._build`."
+        ) from exc
+    p, u64, i32, sz = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t
+    cp = ctypes.c_char_p
+    sig = {
+        "fddm_get_num_devices": [ctypes.POINTER(i32), cp, sz],
+        "fddm_set_device": [i32, cp, sz],
+        "fddm_get_device": [ctypes.POINTER(i32), cp, sz],
+        "fddm_get_free_device_memory": [ctypes.POINTER(u64), cp, sz],
+        "fddm_ddm_fft": [p, i32, u64, u64, u64, p, u64, u64, u64, u64, p, i32, i32, i32, p, cp, sz],
+        "fddm_ddm_diff": [p, i32, u64, u64, u64, p, u64, u64, u64, p, i32, i32, i32, p, cp, sz],
+        "fddm_release_cache": [cp, sz],
+        "fddm_plan_create": [ctypes.POINTER(p), i32, i32, u64, u64, u64, p, u64, u64, u64, u64,
+                             p, i32, i32, i32, cp, sz],
+        "fddm_plan_destroy": [p],
+        "fddm_plan_workspace_size": [p, ctypes.POINTER(u64)],
+        "fddm_plan_output_size": [p, ctypes.POINTER(u64)],
+        "fddm_plan_execute": [p, p, p, p, p, cp, sz],
+        "fddm_plan_timing_begin": [p, i32, cp, sz],
+        "fddm_plan_timing_end": [p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32), cp, sz],
+        "fddm_plan_spectra": [p, p, p, p, cp, sz],
+        "fddm_malloc": [ctypes.POINTER(p), u64, cp, sz],
+        "fddm_free": [p],
+        "fddm_memcpy_htod": [p, p, u64, cp, sz],
+        "fddm_memcpy_dtoh": [p, p, u64, cp, sz],
+        "fddm_stream_create": [ctypes.POINTER(p), cp, sz],
+        "fddm_stream_destroy": [p],
+        "fddm_stream_synchronize": [p, cp, sz],
+        "fddm_test_fft": [p, u64, u64, i32, cp, sz],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = i32
+    _lib = L
+    return L
+
+
+EXPORTED_SYMBOLS = (
+    "fddm_get_num_devices",
+    "fddm_set_device",
+    "fddm_get_device",
+    "fddm_get_free_device_memory",
+    "fddm_ddm_fft",
+    "fddm_ddm_diff",
+    "fddm_release_cache",
+    "fddm_plan_create",
+    "fddm_plan_destroy",
+    "fddm_plan_workspace_size",
+    "fddm_plan_output_size",
+    "fddm_plan_execute",
+    "fddm_plan_timing_begin",
+    "fddm_plan_timing_end",
+    "fddm_plan_spectra",
+    "fddm_malloc",
+    "fddm_free",
+    "fddm_memcpy_htod",
+    "fddm_memcpy_dtoh",
+    "fddm_stream_create",
+    "fddm_stream_destroy",
+    "fddm_stream_synchronize",
+    "fddm_test_fft",
+)
+
+
+def _call(fname: str, *args) -> None:
+    err = ctypes.create_string_buffer(_ERRLEN)
+    rc = getattr(lib(), fname)(*args, err, _ERRLEN)
+    if rc != 0:
+        msg = err.value.decode(errors="replace") or f"error code {rc}"
+        raise RuntimeError(f"{fname}: {msg}")
+
+
+# ---- device utilities (reference ddm_cuda_bindings.h:26-31) ----
+def get_num_devices() -> int:
+    n = ctypes.c_int(0)
+    _call("fddm_get_num_devices", ctypes.byref(n))
+    return n.value
+
+
+def set_device(gpu_id: int) -> None:
+    _call("fddm_set_device", int(gpu_id))
+
+
+def get_device() -> int:
+    d = ctypes.c_int(0)
+    _call("fddm_get_device", ctypes.byref(d))
+    return d.value
+
+
+def get_free_device_memory() -> int:
+    f = ctypes.c_uint64(0)
+    _call("fddm_get_free_device_memory", ctypes.byref(f))
+    return f.value
+
+
+def release_cache() -> None:
+    _call("fddm_release_cache")
+
+
+# ---- argument preparation ----
+def _img_arg(img_seq: np.ndarray) -> np.ndarray:
+    img = np.ascontiguousarray(img_seq)
+    if img.ndim != 3:
+        raise RuntimeError(f"img_seq must be 3D (t, y, x), got shape {img.shape}")
+    if img.dtype not in DTYPE_CODES:
+        raise RuntimeError(f"unsupported image dtype {img.dtype}")
+    return img
+
+
+def _lags_arg(lags) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(lags, dtype=np.uint32))
+
+
+def _window_arg(window: Optional[np.ndarray]):
+    if window is None or np.size(window) == 0:
+        return None, 0, None
+    w = np.ascontiguousarray(window)
+    if w.dtype == np.float32:
+        return w.ctypes.data, 1, w
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    return w.ctypes.data, 0, w
+
+
+def _out_code(out_dtype) -> int:
+    dt = np.dtype(out_dtype)
+    if dt not in OUT_CODES:
+        raise RuntimeError(f"unsupported output dtype {dt}")
+    return OUT_CODES[dt]
+
+
+def _prec_code(precision: str) -> int:
+    if precision not in PRECISIONS:
+        raise RuntimeError(
+            f"Unknown precision '{precision}'. Only possible options are {list(PRECISIONS)}."
+        )
+    return PRECISIONS[precision]
+
+
+def ddm_fft(img_seq, lags, nx, ny, nt, window, precision="fp64", out_dtype=np.float64):
+    """fft mode on the current device; returns (NL+2, ny, nx//2+1) of out_dtype."""
+    img = _img_arg(img_seq)
+    lg = _lags_arg(lags)
+    wp, wf32, _keep = _window_arg(window)
+    out = np.empty((len(lg) + 2, ny, nx // 2 + 1), dtype=out_dtype)
+    L, H, W = img.shape
+    _call(
+        "fddm_ddm_fft", img.ctypes.data, DTYPE_CODES[img.dtype], L, H, W, lg.ctypes.data, len(lg),
+        nx, ny, nt, wp, wf32, _prec_code(precision), _out_code(out_dtype), out.ctypes.data,
+    )
+    return out
+
+
+def ddm_diff(img_seq, lags, nx, ny, window, precision="fp64", out_dtype=np.float64):
+    """diff mode on the current device; returns (NL+2, ny, nx//2+1) of out_dtype."""
+    img = _img_arg(img_seq)
+    lg = _lags_arg(lags)
+    wp, wf32, _keep = _window_arg(window)
+    out = np.empty((len(lg) + 2, ny, nx // 2 + 1), dtype=out_dtype)
+    L, H, W = img.shape
+    _call(
+        "fddm_ddm_diff", img.ctypes.data, DTYPE_CODES[img.dtype], L, H, W, lg.ctypes.data, len(lg),
+        nx, ny, wp, wf32, _prec_code(precision), _out_code(out_dtype), out.ctypes.data,
+    )
+    return out
+
+
+def device_fft(x: np.ndarray, precision: str = "fp64") -> np.ndarray:
+    """Forward FFT along the last axis with the device FFT engine (test entry)."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    n = a.shape[-1]
+    _call("fddm_test_fft", a.ctypes.data, n, a.size // n, _prec_code(precision))
+    return a
+
+
+# ---- device buffers, streams and plans ----
+class DeviceBuffer:
+    """Device allocation owned by Python; freed on ``free()`` or collection."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        _call("fddm_malloc", ctypes.byref(self.ptr), self.nbytes)
+
+    @classmethod
+    def from_array(cls, arr: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        buf = cls(a.nbytes)
+        _call("fddm_memcpy_htod", buf.ptr, a.ctypes.data, a.nbytes)
+        return buf
+
+    def to_array(self, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise RuntimeError("device buffer smaller than requested array")
+        _call("fddm_memcpy_dtoh", out.ctypes.data, self.ptr, out.nbytes)
+        return out
+
+    def free(self) -> None:
+        if self.ptr is not None and self.ptr.value:
+            lib().fddm_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        self.handle = ctypes.c_void_p()
+        _call("fddm_stream_create", ctypes.byref(self.handle))
+
+    def synchronize(self) -> None:
+        _call("fddm_stream_synchronize", self.handle)
+
+    def destroy(self) -> None:
+        if self.handle is not None and self.handle.value:
+            lib().fddm_stream_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Plan:
+    """A reusable execution plan for device-resident stacks of one shape."""
+
+    STAGES = ("frame0_spectra", "row_pass", "column_pass", "time_or_diff")
+
+    def __init__(self, mode, dtype, shape, lags, nx, ny, nt=0, window=None,
+                 precision="fp64", out_dtype=np.float64):
+        self.handle = ctypes.c_void_p()
+        lg = _lags_arg(lags)
+        wp, wf32, _keep = _window_arg(window)
+        L, H, W = shape
+        self.mode = mode
+        self.shape = tuple(shape)
+        self.nl = len(lg)
+        self.nx, self.ny, self.nt = nx, ny, nt
+        self.out_dtype = np.dtype(out_dtype)
+        _call(
+            "fddm_plan_create", ctypes.byref(self.handle), MODE_FFT if mode == "fft" else MODE_DIFF,
+            DTYPE_CODES[np.dtype(dtype)], L, H, W, lg.ctypes.data, len(lg), nx, ny, nt, wp, wf32,
+            _prec_code(precision), _out_code(out_dtype),
+        )
+
+    @property
+    def out_shape(self):
+        return (self.nl + 2, self.ny, self.nx // 2 + 1)
+
+    def workspace_size(self) -> int:
+        b = ctypes.c_uint64(0)
+        lib().fddm_plan_workspace_size(self.handle, ctypes.byref(b))
+        return b.value
+
+    def output_size(self) -> int:
+        b = ctypes.c_uint64(0)
+        lib().fddm_plan_output_size(self.handle, ctypes.byref(b))
+        return b.value
+
+    def execute(self, d_img: DeviceBuffer, d_out: DeviceBuffer, d_ws: DeviceBuffer,
+                stream: Optional[Stream] = None) -> None:
+        _call("fddm_plan_execute", self.handle, d_img.ptr, d_out.ptr, d_ws.ptr,
+              stream.handle if stream is not None else None)
+
+    def timing_begin(self, max_steps: int) -> None:
+        _call("fddm_plan_timing_begin", self.handle, int(max_steps))
+
+    def timing_end(self):
+        ms = (ctypes.c_double * 4)()
+        steps = ctypes.c_int(0)
+        _call("fddm_plan_timing_end", self.handle, ms, ctypes.byref(steps))
+        return dict(zip(self.STAGES, list(ms))), steps.value
+
+    def spectra(self, d_img: DeviceBuffer, d_ws: DeviceBuffer) -> np.ndarray:
+        L, H, W = self.shape
+        nq = self.ny * (self.nx // 2 + 1)
+        out = np.empty((nq, L), dtype=np.complex128)
+        _call("fddm_plan_spectra", self.handle, d_img.ptr, d_ws.ptr, out.ctypes.data)
+        return out
+
+    def destroy(self) -> None:
+        if self.handle is not None and self.handle.value:
+            lib().fddm_plan_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

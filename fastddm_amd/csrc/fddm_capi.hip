@@ -1,0 +1,672 @@
+// C ABI and host orchestration of the image structure function engine.
+//
+// One plan holds the sizes, the mixed-radix factorisations, device twiddle
+// tables, lags and window; execute runs every stage back to back on one
+// stream with no host synchronisation:
+//   [frame-0 spectra] -> row pass -> column pass (q-major) -> time / diff kernel.
+// The host-buffer entry points (fddm_ddm_fft / fddm_ddm_diff) wrap a plan with
+// cached device buffers, one H2D copy of the raw pixels and one D2H copy of
+// the result.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "fddm_hip.h"
+#include "fddm_kernels.hpp"
+
+using namespace fddm;
+
+struct fddm_plan {
+  int mode = MODE_FFT;
+  int precision = PREC_FP64;
+  int in_dtype = IN_U16;
+  int out_dtype = OUT_F64;
+  uint64_t L = 0, H = 0, W = 0, nx = 0, ny = 0, nt = 0, nxh = 0, nq = 0, nl = 0;
+  int Tb = 2, Gx = 1, Gy = 1;
+  uint64_t Lp = 0, nblocks = 0;
+  bool subtract0 = false;
+  FftDesc dx{}, dy{}, dt{};
+  void* d_lags = nullptr;
+  void* d_window = nullptr;
+  void* d_twx = nullptr;
+  void* d_twy = nullptr;
+  void* d_twt = nullptr;
+  uint64_t bytes_R = 0, bytes_S = 0, bytes_S0 = 0, ws_bytes = 0;
+  // timing
+  std::vector<hipEvent_t> events;
+  int timing_cap = 0;
+  int timing_steps = 0;
+};
+
+namespace {
+
+constexpr int kEventsPerStep = 5;
+constexpr size_t kLdsLimit = 160 * 1024 - 1024;
+
+int set_err(char* err, size_t len, int code, const char* fmt, ...) {
+  if (err && len) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, len, fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+int hip_err(char* err, size_t len, hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return set_err(err, len, FDDM_ERR_MEMORY, "Not enough space on GPU (%s: %s)", what, hipGetErrorString(e));
+  return set_err(err, len, FDDM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define FDDM_HIP(expr)                                          \
+  do {                                                          \
+    hipError_t e_ = (expr);                                     \
+    if (e_ != hipSuccess) return hip_err(err, errlen, e_, #expr); \
+  } while (0)
+
+uint64_t align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
+
+size_t in_bytes(int dtype) {
+  switch (dtype) {
+    case IN_U8: return 1;
+    case IN_I16:
+    case IN_U16: return 2;
+    case IN_I32:
+    case IN_U32:
+    case IN_F32: return 4;
+    default: return 8;
+  }
+}
+
+// Radix plan: 2^a as 8s (two 4s or one 4 for the remainder), then 3s, 5s, 7s.
+bool factorize(uint64_t n, FftDesc& d) {
+  std::memset(&d, 0, sizeof(d));
+  if (n == 0 || n > (1u << 20)) return false;
+  d.n = static_cast<int>(n);
+  uint64_t r = n;
+  int a = 0;
+  while (r % 2 == 0) {
+    r /= 2;
+    ++a;
+  }
+  d.c8 = a / 3;
+  const int rem = a % 3;
+  if (rem == 1 && d.c8 > 0) {
+    --d.c8;
+    d.c4 = 2;
+  } else if (rem == 1) {
+    d.c2 = 1;
+  } else if (rem == 2) {
+    d.c4 = 1;
+  }
+  while (r % 3 == 0) {
+    r /= 3;
+    ++d.c3;
+  }
+  while (r % 5 == 0) {
+    r /= 5;
+    ++d.c5;
+  }
+  while (r % 7 == 0) {
+    r /= 7;
+    ++d.c7;
+  }
+  return r == 1;
+}
+
+// w[m] = exp(-2 pi i m / n), evaluated in long double and rounded once.
+template <typename T>
+std::vector<T> twiddles(uint64_t n) {
+  std::vector<T> w(2 * n);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (uint64_t m = 0; m < n; ++m) {
+    const long double ang = -two_pi * static_cast<long double>(m) / static_cast<long double>(n);
+    w[2 * m] = static_cast<T>(cosl(ang));
+    w[2 * m + 1] = static_cast<T>(sinl(ang));
+  }
+  return w;
+}
+
+int upload(void** dst, const void* src, size_t bytes, char* err, size_t errlen) {
+  FDDM_HIP(hipMalloc(dst, bytes ? bytes : 1));
+  if (bytes) FDDM_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  return FDDM_OK;
+}
+
+int upload_twiddles(void** dst, uint64_t n, bool f64, char* err, size_t errlen) {
+  if (f64) {
+    auto w = twiddles<double>(n);
+    return upload(dst, w.data(), w.size() * sizeof(double), err, errlen);
+  }
+  auto w = twiddles<float>(n);
+  return upload(dst, w.data(), w.size() * sizeof(float), err, errlen);
+}
+
+void plan_free_device(fddm_plan* p) {
+  for (void* ptr : {p->d_lags, p->d_window, p->d_twx, p->d_twy, p->d_twt})
+    if (ptr) (void)hipFree(ptr);
+  p->d_lags = p->d_window = p->d_twx = p->d_twy = p->d_twt = nullptr;
+  for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
+  p->events.clear();
+}
+
+int plan_create(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H, uint64_t W,
+                const uint32_t* lags, uint64_t nl, uint64_t nx, uint64_t ny, uint64_t nt,
+                const void* window, int window_f32, int precision, int out_dtype, char* err,
+                size_t errlen) {
+  if (!out) return set_err(err, errlen, FDDM_ERR_ARG, "plan pointer is NULL");
+  *out = nullptr;
+  if (mode != MODE_FFT && mode != MODE_DIFF) return set_err(err, errlen, FDDM_ERR_ARG, "unknown mode %d", mode);
+  if (dtype < IN_U8 || dtype > IN_F64) return set_err(err, errlen, FDDM_ERR_ARG, "unknown input dtype %d", dtype);
+  if (precision < PREC_FP64 || precision > PREC_FP32C)
+    return set_err(err, errlen, FDDM_ERR_ARG, "unknown precision %d", precision);
+  if (out_dtype != OUT_F64 && out_dtype != OUT_F32)
+    return set_err(err, errlen, FDDM_ERR_ARG, "unknown output dtype %d", out_dtype);
+  if (L == 0 || H == 0 || W == 0) return set_err(err, errlen, FDDM_ERR_ARG, "empty image stack");
+  if (nx < W || ny < H) return set_err(err, errlen, FDDM_ERR_ARG, "nx, ny must be >= image width, height");
+  if (nl > 0 && !lags) return set_err(err, errlen, FDDM_ERR_ARG, "lags pointer is NULL");
+  for (uint64_t i = 0; i < nl; ++i) {
+    if (lags[i] < 1 || lags[i] > L - 1)
+      return set_err(err, errlen, FDDM_ERR_ARG, "lag %u outside [1, %llu]", lags[i],
+                     static_cast<unsigned long long>(L - 1));
+    if (i > 0 && lags[i] <= lags[i - 1])
+      return set_err(err, errlen, FDDM_ERR_ARG, "lags must be sorted and unique");
+  }
+  if (L > (1ull << 30) || nl > (1ull << 30)) return set_err(err, errlen, FDDM_ERR_SIZE, "too many frames");
+
+  auto* p = new (std::nothrow) fddm_plan();
+  if (!p) return set_err(err, errlen, FDDM_ERR_MEMORY, "host allocation failed");
+  p->mode = mode;
+  p->precision = precision;
+  p->in_dtype = dtype;
+  p->out_dtype = out_dtype;
+  p->L = L;
+  p->H = H;
+  p->W = W;
+  p->nx = nx;
+  p->ny = ny;
+  p->nt = mode == MODE_FFT ? nt : 0;
+  p->nxh = nx / 2 + 1;
+  p->nq = p->nxh * ny;
+  p->nl = nl;
+  p->subtract0 = precision != PREC_FP32;
+  const bool f64 = precision == PREC_FP64;
+  const size_t csz = f64 ? 16 : 8;
+
+  auto fail = [&](int code) {
+    plan_free_device(p);
+    delete p;
+    return code;
+  };
+  if (!factorize(nx, p->dx) || !factorize(ny, p->dy))
+    return fail(set_err(err, errlen, FDDM_ERR_SIZE, "nx=%llu, ny=%llu must be 2^a 3^b 5^c 7^d",
+                        (unsigned long long)nx, (unsigned long long)ny));
+  if (p->nq > (1ull << 31)) return fail(set_err(err, errlen, FDDM_ERR_SIZE, "image too large"));
+  // Frames per block: runs of Tb * sizeof(complex) = 128 B in the q-major
+  // store, halved while a column slab (ny x Tb complex) exceeds 64 KiB.
+  p->Tb = f64 ? 8 : 16;
+  while (p->Tb > 2 && ny * p->Tb * csz > 64 * 1024) p->Tb /= 2;
+  if (ny * p->Tb * csz > kLdsLimit)
+    return fail(set_err(err, errlen, FDDM_ERR_SIZE, "ny=%llu too large for the column pass", (unsigned long long)ny));
+  // Transforms handled at once by the row / column kernels (<= 16 or 32
+  // elements per thread).
+  const uint64_t span = 4096;
+  p->Gx = 1;
+  while (p->Gx * 2 <= p->Tb / 2 && (p->Gx * 2) * nx <= span) p->Gx *= 2;
+  p->Gy = 1;
+  while (p->Gy * 2 <= p->Tb && (p->Gy * 2) * ny <= span) p->Gy *= 2;
+  if (fft_bucket(static_cast<long long>(p->Gx) * nx) == 0 || fft_bucket(p->Gx * nx) > 32 ||
+      fft_bucket(static_cast<long long>(p->Gy) * ny) > 32 || fft_bucket(p->Gy * ny) == 0)
+    return fail(set_err(err, errlen, FDDM_ERR_SIZE, "nx or ny above 8192"));
+  if (p->Gx * (nx + 1) * csz > kLdsLimit)
+    return fail(set_err(err, errlen, FDDM_ERR_SIZE, "nx=%llu too large for the row pass", (unsigned long long)nx));
+  p->nblocks = (L + p->Tb - 1) / p->Tb;
+  p->Lp = p->nblocks * p->Tb;
+
+  if (mode == MODE_FFT) {
+    if (nt < 2 * L - 1) return fail(set_err(err, errlen, FDDM_ERR_ARG, "nt must be >= 2*length-1"));
+    if (!factorize(nt, p->dt))
+      return fail(set_err(err, errlen, FDDM_ERR_SIZE, "nt=%llu must be 2^a 3^b 5^c 7^d", (unsigned long long)nt));
+    const int e = fft_bucket(static_cast<long long>(nt));
+    if (e == 0 || (f64 && e > 32) || nt * csz > kLdsLimit)
+      return fail(set_err(err, errlen, FDDM_ERR_SIZE,
+                          "nt=%llu too long for the LDS-resident time kernel (max %d in %s)",
+                          (unsigned long long)nt, f64 ? 8192 : 16384, f64 ? "fp64" : "fp32"));
+    // k_time keeps ceil(L / 256) prefix sums per thread in a register array
+    // of E / 2 entries.
+    if ((L + 255) / 256 > static_cast<uint64_t>(e / 2))
+      return fail(set_err(err, errlen, FDDM_ERR_SIZE, "length %llu too large for nt=%llu",
+                          (unsigned long long)L, (unsigned long long)nt));
+  }
+
+  int rc;
+  if ((rc = upload_twiddles(&p->d_twx, nx, f64, err, errlen))) return fail(rc);
+  if ((rc = upload_twiddles(&p->d_twy, ny, f64, err, errlen))) return fail(rc);
+  if (mode == MODE_FFT && (rc = upload_twiddles(&p->d_twt, nt, f64, err, errlen))) return fail(rc);
+  if ((rc = upload(&p->d_lags, lags, nl * sizeof(uint32_t), err, errlen))) return fail(rc);
+  if (window) {
+    const size_t npx = H * W;
+    if (f64) {
+      std::vector<double> w(npx);
+      for (size_t i = 0; i < npx; ++i)
+        w[i] = window_f32 ? static_cast<const float*>(window)[i] : static_cast<const double*>(window)[i];
+      rc = upload(&p->d_window, w.data(), npx * sizeof(double), err, errlen);
+    } else {
+      std::vector<float> w(npx);
+      for (size_t i = 0; i < npx; ++i)
+        w[i] = window_f32 ? static_cast<const float*>(window)[i]
+                          : static_cast<float>(static_cast<const double*>(window)[i]);
+      rc = upload(&p->d_window, w.data(), npx * sizeof(float), err, errlen);
+    }
+    if (rc) return fail(rc);
+  }
+  p->bytes_R = align256(p->nblocks * p->nxh * H * p->Tb * csz);
+  p->bytes_S = align256(p->nq * p->Lp * csz);
+  p->bytes_S0 = p->subtract0 ? align256(p->nq * p->Tb * csz) : 0;
+  p->ws_bytes = p->bytes_R + p->bytes_S + p->bytes_S0;
+  *out = p;
+  return FDDM_OK;
+}
+
+uint64_t output_bytes(const fddm_plan* p) {
+  return (p->nl + 2) * p->nq * (p->out_dtype == OUT_F32 ? 4 : 8);
+}
+
+int record(fddm_plan* p, int slot, hipStream_t s, char* err, size_t errlen) {
+  if (p->timing_cap == 0 || p->timing_steps >= p->timing_cap) return FDDM_OK;
+  FDDM_HIP(hipEventRecord(p->events[p->timing_steps * kEventsPerStep + slot], s));
+  return FDDM_OK;
+}
+
+int run_spatial(fddm_plan* p, const void* d_img, void* ws, hipStream_t s, char* err, size_t errlen) {
+  auto* base = static_cast<unsigned char*>(ws);
+  void* R = base;
+  void* S = base + p->bytes_R;
+  void* S0 = base + p->bytes_R + p->bytes_S;
+  const double norm = 1.0 / std::sqrt(static_cast<double>(p->nx) * static_cast<double>(p->ny));
+  RowArgs ra{};
+  ra.img = d_img;
+  ra.in_dtype = p->in_dtype;
+  ra.L = p->L;
+  ra.H = static_cast<int>(p->H);
+  ra.W = static_cast<int>(p->W);
+  ra.nx = static_cast<int>(p->nx);
+  ra.Tb = p->Tb;
+  ra.G = p->Gx;
+  ra.window = p->d_window;
+  ra.norm = norm;
+  ra.dx = p->dx;
+  ra.twx = p->d_twx;
+  ra.R = R;
+  ColArgs ca{};
+  ca.R = R;
+  ca.H = static_cast<int>(p->H);
+  ca.ny = static_cast<int>(p->ny);
+  ca.nxh = static_cast<int>(p->nxh);
+  ca.Tb = p->Tb;
+  ca.G = p->Gy;
+  ca.dy = p->dy;
+  ca.twy = p->d_twy;
+  int rc;
+  if ((rc = record(p, 0, s, err, errlen))) return rc;
+  if (p->subtract0) {
+    // F(0) of the raw first frame, for the power spectrum.
+    ra.frames = 1;
+    ra.subtract0 = 0;
+    FDDM_HIP(launch_rows(ra, p->precision, s));
+    ca.nblocks = 1;
+    ca.Lp = static_cast<uint64_t>(p->Tb);
+    ca.S = S0;
+    FDDM_HIP(launch_cols(ca, p->precision, s));
+  }
+  if ((rc = record(p, 1, s, err, errlen))) return rc;
+  ra.frames = p->L;
+  ra.subtract0 = p->subtract0 ? 1 : 0;
+  FDDM_HIP(launch_rows(ra, p->precision, s));
+  if ((rc = record(p, 2, s, err, errlen))) return rc;
+  ca.nblocks = p->nblocks;
+  ca.Lp = p->Lp;
+  ca.S = S;
+  FDDM_HIP(launch_cols(ca, p->precision, s));
+  if ((rc = record(p, 3, s, err, errlen))) return rc;
+  return FDDM_OK;
+}
+
+int plan_execute(fddm_plan* p, const void* d_img, void* d_out, void* ws, hipStream_t s, char* err,
+                 size_t errlen) {
+  if (!p || !d_img || !d_out || !ws) return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument to execute");
+  int rc = run_spatial(p, d_img, ws, s, err, errlen);
+  if (rc) return rc;
+  auto* base = static_cast<unsigned char*>(ws);
+  TimeArgs ta{};
+  ta.S = base + p->bytes_R;
+  ta.Lp = p->Lp;
+  ta.L = static_cast<int>(p->L);
+  ta.ny = static_cast<int>(p->ny);
+  ta.nxh = static_cast<int>(p->nxh);
+  ta.S0 = p->subtract0 ? base + p->bytes_R + p->bytes_S : nullptr;
+  ta.s0_stride = static_cast<uint64_t>(p->Tb);
+  ta.lags = static_cast<const uint32_t*>(p->d_lags);
+  ta.nl = static_cast<int>(p->nl);
+  ta.dt = p->dt;
+  ta.twt = p->d_twt;
+  ta.out = d_out;
+  if (p->mode == MODE_FFT) {
+    FDDM_HIP(launch_time(ta, p->precision, p->out_dtype, s));
+  } else {
+    FDDM_HIP(launch_diff(ta, p->precision, p->out_dtype, s));
+  }
+  if ((rc = record(p, 4, s, err, errlen))) return rc;
+  if (p->timing_cap > 0 && p->timing_steps < p->timing_cap) ++p->timing_steps;
+  return FDDM_OK;
+}
+
+// Device buffers reused across the host-buffer entry points, per device.
+struct HostPathCache {
+  void* img = nullptr;
+  uint64_t img_cap = 0;
+  void* out = nullptr;
+  uint64_t out_cap = 0;
+  void* ws = nullptr;
+  uint64_t ws_cap = 0;
+  hipStream_t stream = nullptr;
+};
+std::mutex g_cache_mu;
+HostPathCache g_cache[64];
+
+int grow(void** ptr, uint64_t* cap, uint64_t need, char* err, size_t errlen) {
+  if (*cap >= need && *ptr) return FDDM_OK;
+  if (*ptr) (void)hipFree(*ptr);
+  *ptr = nullptr;
+  *cap = 0;
+  FDDM_HIP(hipMalloc(ptr, need ? need : 1));
+  *cap = need;
+  return FDDM_OK;
+}
+
+int host_path(int mode, const void* img, int dtype, uint64_t L, uint64_t H, uint64_t W,
+              const uint32_t* lags, uint64_t nl, uint64_t nx, uint64_t ny, uint64_t nt,
+              const void* window, int window_f32, int precision, int out_dtype, void* out,
+              char* err, size_t errlen) {
+  if (!img || !out) return set_err(err, errlen, FDDM_ERR_ARG, "NULL image or output pointer");
+  int dev = 0;
+  FDDM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(err, errlen, FDDM_ERR_DEVICE, "device id %d not supported", dev);
+  fddm_plan* p = nullptr;
+  int rc = plan_create(&p, mode, dtype, L, H, W, lags, nl, nx, ny, nt, window, window_f32, precision,
+                       out_dtype, err, errlen);
+  if (rc) return rc;
+  struct PlanGuard {
+    fddm_plan* p;
+    ~PlanGuard() {
+      plan_free_device(p);
+      delete p;
+    }
+  } guard{p};
+  const uint64_t img_bytes = L * H * W * in_bytes(dtype);
+  const uint64_t out_b = output_bytes(p);
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  HostPathCache& c = g_cache[dev];
+  size_t free_b = 0, total_b = 0;
+  FDDM_HIP(hipMemGetInfo(&free_b, &total_b));
+  const uint64_t need = (c.img_cap >= img_bytes ? 0 : img_bytes) + (c.out_cap >= out_b ? 0 : out_b) +
+                        (c.ws_cap >= p->ws_bytes ? 0 : p->ws_bytes);
+  if (need > static_cast<uint64_t>(0.95 * static_cast<double>(free_b)))
+    return set_err(err, errlen, FDDM_ERR_MEMORY,
+                   "Not enough space on GPU: %llu bytes needed, %llu free",
+                   static_cast<unsigned long long>(need), static_cast<unsigned long long>(free_b));
+  if (!c.stream) FDDM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  if ((rc = grow(&c.img, &c.img_cap, img_bytes, err, errlen))) return rc;
+  if ((rc = grow(&c.out, &c.out_cap, out_b, err, errlen))) return rc;
+  if ((rc = grow(&c.ws, &c.ws_cap, p->ws_bytes, err, errlen))) return rc;
+  FDDM_HIP(hipMemcpyAsync(c.img, img, img_bytes, hipMemcpyHostToDevice, c.stream));
+  if ((rc = plan_execute(p, c.img, c.out, c.ws, c.stream, err, errlen))) return rc;
+  FDDM_HIP(hipMemcpyAsync(out, c.out, out_b, hipMemcpyDeviceToHost, c.stream));
+  FDDM_HIP(hipStreamSynchronize(c.stream));
+  return FDDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fddm_get_num_devices(int* count, char* err, size_t errlen) {
+  if (!count) return set_err(err, errlen, FDDM_ERR_ARG, "NULL count");
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e == hipErrorNoDevice) {
+    *count = 0;
+    return FDDM_OK;
+  }
+  if (e != hipSuccess) return hip_err(err, errlen, e, "hipGetDeviceCount");
+  return FDDM_OK;
+}
+
+int fddm_set_device(int gpu_id, char* err, size_t errlen) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess && e != hipErrorNoDevice) return hip_err(err, errlen, e, "hipGetDeviceCount");
+  if (gpu_id < 0 || gpu_id >= n)
+    return set_err(err, errlen, FDDM_ERR_DEVICE, "Device id %d out of range [0, %d)", gpu_id, n);
+  FDDM_HIP(hipSetDevice(gpu_id));
+  return FDDM_OK;
+}
+
+int fddm_get_device(int* gpu_id, char* err, size_t errlen) {
+  if (!gpu_id) return set_err(err, errlen, FDDM_ERR_ARG, "NULL gpu_id");
+  FDDM_HIP(hipGetDevice(gpu_id));
+  return FDDM_OK;
+}
+
+int fddm_get_free_device_memory(uint64_t* free_bytes, char* err, size_t errlen) {
+  if (!free_bytes) return set_err(err, errlen, FDDM_ERR_ARG, "NULL free_bytes");
+  size_t f = 0, t = 0;
+  FDDM_HIP(hipMemGetInfo(&f, &t));
+  *free_bytes = f;
+  return FDDM_OK;
+}
+
+int fddm_ddm_fft(const void* img, int dtype, uint64_t length, uint64_t height, uint64_t width,
+                 const uint32_t* lags, uint64_t num_lags, uint64_t nx, uint64_t ny, uint64_t nt,
+                 const void* window, int window_f32, int precision, int out_dtype, void* out, char* err,
+                 size_t errlen) {
+  return host_path(MODE_FFT, img, dtype, length, height, width, lags, num_lags, nx, ny, nt, window,
+                   window_f32, precision, out_dtype, out, err, errlen);
+}
+
+int fddm_ddm_diff(const void* img, int dtype, uint64_t length, uint64_t height, uint64_t width,
+                  const uint32_t* lags, uint64_t num_lags, uint64_t nx, uint64_t ny, const void* window,
+                  int window_f32, int precision, int out_dtype, void* out, char* err, size_t errlen) {
+  return host_path(MODE_DIFF, img, dtype, length, height, width, lags, num_lags, nx, ny, 0, window,
+                   window_f32, precision, out_dtype, out, err, errlen);
+}
+
+int fddm_release_cache(char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  int dev = 0;
+  FDDM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return FDDM_OK;
+  HostPathCache& c = g_cache[dev];
+  if (c.stream) FDDM_HIP(hipStreamSynchronize(c.stream));
+  for (void* ptr : {c.img, c.out, c.ws})
+    if (ptr) (void)hipFree(ptr);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  c = HostPathCache{};
+  return FDDM_OK;
+}
+
+int fddm_plan_create(fddm_plan** plan, int mode, int dtype, uint64_t length, uint64_t height, uint64_t width,
+                     const uint32_t* lags, uint64_t num_lags, uint64_t nx, uint64_t ny, uint64_t nt,
+                     const void* window, int window_f32, int precision, int out_dtype, char* err,
+                     size_t errlen) {
+  return plan_create(plan, mode, dtype, length, height, width, lags, num_lags, nx, ny, nt, window, window_f32,
+                     precision, out_dtype, err, errlen);
+}
+
+int fddm_plan_destroy(fddm_plan* plan) {
+  if (!plan) return FDDM_OK;
+  plan_free_device(plan);
+  delete plan;
+  return FDDM_OK;
+}
+
+int fddm_plan_workspace_size(const fddm_plan* plan, uint64_t* bytes) {
+  if (!plan || !bytes) return FDDM_ERR_ARG;
+  *bytes = plan->ws_bytes;
+  return FDDM_OK;
+}
+
+int fddm_plan_output_size(const fddm_plan* plan, uint64_t* bytes) {
+  if (!plan || !bytes) return FDDM_ERR_ARG;
+  *bytes = output_bytes(plan);
+  return FDDM_OK;
+}
+
+int fddm_plan_execute(fddm_plan* plan, const void* d_img, void* d_out, void* d_workspace, void* stream,
+                      char* err, size_t errlen) {
+  return plan_execute(plan, d_img, d_out, d_workspace, static_cast<hipStream_t>(stream), err, errlen);
+}
+
+int fddm_plan_timing_begin(fddm_plan* plan, int max_steps, char* err, size_t errlen) {
+  if (!plan || max_steps <= 0 || max_steps > 100000)
+    return set_err(err, errlen, FDDM_ERR_ARG, "bad timing request");
+  for (hipEvent_t e : plan->events) (void)hipEventDestroy(e);
+  plan->events.assign(static_cast<size_t>(max_steps) * kEventsPerStep, nullptr);
+  for (auto& e : plan->events) FDDM_HIP(hipEventCreate(&e));
+  plan->timing_cap = max_steps;
+  plan->timing_steps = 0;
+  return FDDM_OK;
+}
+
+int fddm_plan_timing_end(fddm_plan* plan, double* stage_ms, int* steps, char* err, size_t errlen) {
+  if (!plan || !stage_ms || !steps) return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument");
+  for (int k = 0; k < 4; ++k) stage_ms[k] = 0.0;
+  *steps = plan->timing_steps;
+  for (int i = 0; i < plan->timing_steps; ++i) {
+    hipEvent_t* ev = &plan->events[static_cast<size_t>(i) * kEventsPerStep];
+    FDDM_HIP(hipEventSynchronize(ev[kEventsPerStep - 1]));
+    for (int k = 0; k < 4; ++k) {
+      float ms = 0.f;
+      FDDM_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      stage_ms[k] += ms;
+    }
+  }
+  for (hipEvent_t e : plan->events) (void)hipEventDestroy(e);
+  plan->events.clear();
+  plan->timing_cap = 0;
+  plan->timing_steps = 0;
+  return FDDM_OK;
+}
+
+int fddm_plan_spectra(fddm_plan* plan, const void* d_img, void* d_workspace, double* host_spectra, char* err,
+                      size_t errlen) {
+  if (!plan || !d_img || !d_workspace || !host_spectra)
+    return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument");
+  const int cap = plan->timing_cap;
+  plan->timing_cap = 0;
+  int rc = run_spatial(plan, d_img, d_workspace, nullptr, err, errlen);
+  plan->timing_cap = cap;
+  if (rc) return rc;
+  FDDM_HIP(hipDeviceSynchronize());
+  const bool f64 = plan->precision == PREC_FP64;
+  const size_t csz = f64 ? 16 : 8;
+  std::vector<unsigned char> tmp(plan->nq * plan->Lp * csz);
+  FDDM_HIP(hipMemcpy(tmp.data(), static_cast<unsigned char*>(d_workspace) + plan->bytes_R, tmp.size(),
+                     hipMemcpyDeviceToHost));
+  for (uint64_t q = 0; q < plan->nq; ++q)
+    for (uint64_t t = 0; t < plan->L; ++t) {
+      const uint64_t i = q * plan->Lp + t, o = 2 * (q * plan->L + t);
+      if (f64) {
+        host_spectra[o] = reinterpret_cast<const double*>(tmp.data())[2 * i];
+        host_spectra[o + 1] = reinterpret_cast<const double*>(tmp.data())[2 * i + 1];
+      } else {
+        host_spectra[o] = reinterpret_cast<const float*>(tmp.data())[2 * i];
+        host_spectra[o + 1] = reinterpret_cast<const float*>(tmp.data())[2 * i + 1];
+      }
+    }
+  return FDDM_OK;
+}
+
+int fddm_malloc(void** ptr, uint64_t bytes, char* err, size_t errlen) {
+  if (!ptr) return set_err(err, errlen, FDDM_ERR_ARG, "NULL ptr");
+  FDDM_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return FDDM_OK;
+}
+
+int fddm_free(void* ptr) { return ptr && hipFree(ptr) != hipSuccess ? FDDM_ERR_HIP : FDDM_OK; }
+
+int fddm_memcpy_htod(void* dst, const void* src, uint64_t bytes, char* err, size_t errlen) {
+  FDDM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return FDDM_OK;
+}
+
+int fddm_memcpy_dtoh(void* dst, const void* src, uint64_t bytes, char* err, size_t errlen) {
+  FDDM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return FDDM_OK;
+}
+
+int fddm_stream_create(void** stream, char* err, size_t errlen) {
+  if (!stream) return set_err(err, errlen, FDDM_ERR_ARG, "NULL stream");
+  hipStream_t s = nullptr;
+  FDDM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return FDDM_OK;
+}
+
+int fddm_stream_destroy(void* stream) {
+  return stream && hipStreamDestroy(static_cast<hipStream_t>(stream)) != hipSuccess ? FDDM_ERR_HIP : FDDM_OK;
+}
+
+int fddm_stream_synchronize(void* stream, char* err, size_t errlen) {
+  FDDM_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return FDDM_OK;
+}
+
+int fddm_test_fft(double* data, uint64_t n, uint64_t batch, int precision, char* err, size_t errlen) {
+  if (!data || n == 0 || batch == 0) return set_err(err, errlen, FDDM_ERR_ARG, "bad FFT test arguments");
+  FftDesc d{};
+  if (!factorize(n, d)) return set_err(err, errlen, FDDM_ERR_SIZE, "n=%llu is not 7-smooth", (unsigned long long)n);
+  const bool f64 = precision == PREC_FP64;
+  const size_t csz = f64 ? 16 : 8;
+  const int e = fft_bucket(static_cast<long long>(n));
+  if (e == 0 || (f64 && e > 32) || n * csz > kLdsLimit) return set_err(err, errlen, FDDM_ERR_SIZE, "n too large");
+  if (batch > (1u << 30)) return set_err(err, errlen, FDDM_ERR_SIZE, "batch too large");
+  void* tw = nullptr;
+  int rc = upload_twiddles(&tw, n, f64, err, errlen);
+  if (rc) return rc;
+  void* buf = nullptr;
+  hipError_t he = hipMalloc(&buf, n * batch * csz);
+  if (he != hipSuccess) {
+    (void)hipFree(tw);
+    return hip_err(err, errlen, he, "hipMalloc");
+  }
+  std::vector<float> f;
+  if (f64) {
+    he = hipMemcpy(buf, data, n * batch * csz, hipMemcpyHostToDevice);
+  } else {
+    f.resize(2 * n * batch);
+    for (size_t i = 0; i < f.size(); ++i) f[i] = static_cast<float>(data[i]);
+    he = hipMemcpy(buf, f.data(), n * batch * csz, hipMemcpyHostToDevice);
+  }
+  if (he == hipSuccess) he = launch_fft_test(buf, static_cast<int>(n), static_cast<int>(batch), d, tw, precision, nullptr);
+  if (he == hipSuccess) he = hipDeviceSynchronize();
+  if (he == hipSuccess) {
+    if (f64) {
+      he = hipMemcpy(data, buf, n * batch * csz, hipMemcpyDeviceToHost);
+    } else {
+      he = hipMemcpy(f.data(), buf, n * batch * csz, hipMemcpyDeviceToHost);
+      for (size_t i = 0; i < f.size(); ++i) data[i] = f[i];
+    }
+  }
+  (void)hipFree(buf);
+  (void)hipFree(tw);
+  if (he != hipSuccess) return hip_err(err, errlen, he, "fft test");
+  return FDDM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,558 @@
+// Kernels of the image structure function path (gfx950, wave64).
+//
+//   k_rows  stage (a), row pass: pixel convert (+ frame-0 subtraction) x window,
+//           two frames packed as one complex row, length-nx FFT, split into the
+//           two half spectra, blocked store R[tb][kx][y][ti].
+//   k_cols  stage (a) column pass + stage (b) transpose: length-ny FFT of Tb
+//           frames of one kx column at once, store q-major S[q][t] in runs of
+//           Tb consecutive frames.
+//   k_time  stage (c) + (e): per-q Wiener-Khinchin correlation (zero-padded
+//           length-nt FFT, |.|^2, second forward FFT), prefix sums of |F|^2,
+//           D(q, dt) for the requested lags, PS and VAR, y-shifted store.
+//   k_diff  stage (d) + (e): per-q direct lag differences, PS and VAR.
+//
+// Semantics follow the reference cores: spectra and normalisation
+// src/cpp/ddm.cc:288-323, correlation and two-ended sums ddm.cc:344-426,
+// differences ddm.cc:117-171, output shift src/cpp/helper_ddm.cc:41-103.
+#include <type_traits>
+
+#include "fddm_fft.hpp"
+#include "fddm_kernels.hpp"
+
+namespace fddm {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+int fft_bucket(long long work) {
+  const long long e = (work + kThreads - 1) / kThreads;
+  if (e <= 4) return 4;
+  if (e <= 8) return 8;
+  if (e <= 16) return 16;
+  if (e <= 32) return 32;
+  if (e <= 64) return 64;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Wave / block reductions (wave64).
+// ---------------------------------------------------------------------------
+template <typename A>
+__device__ __forceinline__ A wave_sum(A v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename A>
+__device__ __forceinline__ A wave_incl_scan(A v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const A u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+template <typename A>
+__device__ __forceinline__ A block_sum(A v) {
+  __shared__ A sh[kWaves];
+  const A w = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = w;
+  __syncthreads();
+  A tot = 0;
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) tot += sh[i];
+  __syncthreads();
+  return tot;
+}
+
+// Exclusive block scan; `total` receives the sum over the block.
+template <typename A>
+__device__ __forceinline__ A block_excl_scan(A v, A& total) {
+  __shared__ A sh[kWaves];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const A inc = wave_incl_scan(v);
+  A exc = __shfl_up(inc, 1, 64);
+  if (lane == 0) exc = 0;
+  if (lane == 63) sh[wid] = inc;
+  __syncthreads();
+  A off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) {
+    const A s = sh[i];
+    if (i < wid) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return off + exc;
+}
+
+// ---------------------------------------------------------------------------
+// Pixel conversion.  Integer frame differences are formed exactly in integer
+// arithmetic before the conversion to T.
+// ---------------------------------------------------------------------------
+template <typename Tin, typename T>
+__device__ __forceinline__ T load_pixel(const Tin* __restrict__ img, size_t idx, size_t idx0, bool sub) {
+  if constexpr (std::is_integral_v<Tin> && sizeof(Tin) <= 2) {
+    int v = static_cast<int>(img[idx]);
+    if (sub) v -= static_cast<int>(img[idx0]);
+    return static_cast<T>(v);
+  } else if constexpr (std::is_integral_v<Tin> && sizeof(Tin) == 4) {
+    long long v = static_cast<long long>(img[idx]);
+    if (sub) v -= static_cast<long long>(img[idx0]);
+    return static_cast<T>(v);
+  } else if constexpr (std::is_integral_v<Tin>) {
+    double v = static_cast<double>(img[idx]);
+    if (sub) v -= static_cast<double>(img[idx0]);
+    return static_cast<T>(v);
+  } else {
+    Tin v = img[idx];
+    if (sub) v -= img[idx0];
+    return static_cast<T>(v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage (a): row pass.
+// ---------------------------------------------------------------------------
+template <typename Tin, typename T, int E>
+__global__ void __launch_bounds__(kThreads) k_rows(RowArgs a) {
+  using C = cplx_t<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* buf = reinterpret_cast<C*>(smem);
+  const int tid = threadIdx.x;
+  const int H = a.H, W = a.W, nx = a.nx, Tb = a.Tb, G = a.G;
+  const int nxh = nx / 2 + 1;
+  const int y = blockIdx.x % H;
+  const uint64_t tb = blockIdx.x / H;
+  const uint64_t t0 = tb * Tb;
+  const Tin* __restrict__ img = static_cast<const Tin*>(a.img);
+  const T* __restrict__ win = static_cast<const T*>(a.window);
+  const T half_norm = static_cast<T>(0.5 * a.norm);
+  const int bstride = nx + 1;
+  const size_t plane = static_cast<size_t>(H) * W;
+  C* __restrict__ R = static_cast<C*>(a.R);
+  const bool sub = a.subtract0 != 0;
+  const C* __restrict__ tw = static_cast<const C*>(a.twx);
+
+  for (int g0 = 0; g0 < Tb / 2; g0 += G) {
+    for (int w = tid; w < G * nx; w += kThreads) {
+      const int p = w / nx, x = w - p * nx;
+      T va = 0, vb = 0;
+      if (x < W) {
+        const uint64_t ta = t0 + 2 * (g0 + p), tb2 = ta + 1;
+        const size_t i0 = static_cast<size_t>(y) * W + x;
+        if (ta < a.frames) va = load_pixel<Tin, T>(img, ta * plane + i0, i0, sub);
+        if (tb2 < a.frames) vb = load_pixel<Tin, T>(img, tb2 * plane + i0, i0, sub);
+        if (win) {
+          const T wv = win[i0];
+          va *= wv;
+          vb *= wv;
+        }
+      }
+      buf[p * bstride + x] = C{va, vb};
+    }
+    __syncthreads();
+    fft_lds<T, E, false>(buf, a.dx, G, bstride, 1, tw, tid, kThreads);
+    // Z = A + i B with A, B the spectra of the two real rows:
+    // A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / 2i.
+    const int nti = 2 * G;
+    for (int w = tid; w < nxh * nti; w += kThreads) {
+      const int tl = w % nti, k = w / nti;
+      const int p = tl >> 1;
+      const C zk = buf[p * bstride + k];
+      const C zn = buf[p * bstride + (k == 0 ? 0 : nx - k)];
+      C o;
+      if ((tl & 1) == 0) {
+        o = C{(zk.x + zn.x) * half_norm, (zk.y - zn.y) * half_norm};
+      } else {
+        o = C{(zk.y + zn.y) * half_norm, (zn.x - zk.x) * half_norm};
+      }
+      const int ti = 2 * g0 + tl;
+      R[((tb * nxh + k) * static_cast<size_t>(H) + y) * Tb + ti] = o;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage (a) columns + (b) q-major store.
+// ---------------------------------------------------------------------------
+template <typename T, int E>
+__global__ void __launch_bounds__(kThreads) k_cols(ColArgs a) {
+  using C = cplx_t<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* buf = reinterpret_cast<C*>(smem);
+  const int tid = threadIdx.x;
+  const int nxh = a.nxh, H = a.H, ny = a.ny, Tb = a.Tb;
+  const int kx = blockIdx.x % nxh;
+  const uint64_t tb = blockIdx.x / nxh;
+  const C* __restrict__ src = static_cast<const C*>(a.R) + (tb * nxh + kx) * static_cast<size_t>(H) * Tb;
+  const int nin = H * Tb, nall = ny * Tb;
+  for (int w = tid; w < nall; w += kThreads) buf[w] = w < nin ? src[w] : C{0, 0};
+  __syncthreads();
+  const C* __restrict__ tw = static_cast<const C*>(a.twy);
+  for (int g0 = 0; g0 < Tb; g0 += a.G) fft_lds<T, E, true>(buf + g0, a.dy, a.G, 1, Tb, tw, tid, kThreads);
+  C* __restrict__ S = static_cast<C*>(a.S);
+  for (int w = tid; w < nall; w += kThreads) {
+    const int ti = w % Tb, ky = w / Tb;
+    S[(static_cast<size_t>(ky) * nxh + kx) * a.Lp + tb * Tb + ti] = buf[w];
+  }
+}
+
+// Output position of half-plane index q = ky * nxh + kx after the fftshift
+// along y: row (ky + ny/2) mod ny.
+__device__ __forceinline__ size_t shifted_q(int q, int ny, int nxh) {
+  const int ky = q / nxh, kx = q - ky * nxh;
+  const int oy = (ky + ny / 2) % ny;
+  return static_cast<size_t>(oy) * nxh + kx;
+}
+
+// ---------------------------------------------------------------------------
+// Stage (c)+(e): Wiener-Khinchin per q.
+// ---------------------------------------------------------------------------
+template <typename T, typename OutT, int E, int PREC>
+__global__ void __launch_bounds__(kThreads) k_time(TimeArgs a, int q_per_xcd, int nq) {
+  using C = cplx_t<T>;
+  using Acc = std::conditional_t<PREC == PREC_FP32, float, double>;
+  constexpr bool kSub = PREC != PREC_FP32;
+  constexpr int SEG = E / 2 > 0 ? E / 2 : 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* buf = reinterpret_cast<C*>(smem);
+  const int tid = threadIdx.x;
+  // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+  // contiguous range of q so neighbouring output columns share its L2.
+  const int q = (blockIdx.x & 7) * q_per_xcd + (blockIdx.x >> 3);
+  if (q >= nq) return;
+  const int L = a.L, n = a.dt.n;
+  const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(q) * a.Lp;
+  for (int t = tid; t < n; t += kThreads) buf[t] = t < L ? src[t] : C{0, 0};
+  __syncthreads();
+
+  const int seg = (L + kThreads - 1) / kThreads;
+  const int t0 = min(L, tid * seg), t1 = min(L, t0 + seg);
+  double mr = 0.0, mi = 0.0;
+  if constexpr (kSub) {
+    for (int t = t0; t < t1; ++t) {
+      mr += static_cast<double>(buf[t].x);
+      mi += static_cast<double>(buf[t].y);
+    }
+    mr = block_sum(mr) / L;
+    mi = block_sum(mi) / L;
+  }
+  const T m_r = static_cast<T>(mr), m_i = static_cast<T>(mi);
+  Acc pre[SEG];
+  Acc run = 0, sr = 0, si = 0;
+#pragma unroll
+  for (int i = 0; i < SEG; ++i) {
+    const int t = t0 + i;
+    if (t < t1) {
+      C f = buf[t];
+      if constexpr (kSub) {
+        f.x -= m_r;
+        f.y -= m_i;
+        buf[t] = f;
+      }
+      run += static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
+      sr += static_cast<Acc>(f.x);
+      si += static_cast<Acc>(f.y);
+      pre[i] = run;
+    }
+  }
+  Acc ptot;
+  const Acc off = block_excl_scan(run, ptot);
+  sr = block_sum(sr);
+  si = block_sum(si);
+
+  const C* __restrict__ tw = static_cast<const C*>(a.twt);
+  fft_lds<T, E, false>(buf, a.dt, 1, n, 1, tw, tid, kThreads);
+  for (int t = tid; t < n; t += kThreads) {
+    const C g = buf[t];
+    buf[t] = C{g.x * g.x + g.y * g.y, 0};
+  }
+  __syncthreads();
+  fft_lds<T, E, false>(buf, a.dt, 1, n, 1, tw, tid, kThreads);
+  // Re buf[k] = n * C(k), C(k) = Re sum_t conj F(t) F(t + k).  Prefix sums
+  // psum(k) = sum_{t<k} |F(t)|^2, k = 1 .. L-1, go to the free tail buf[L..n).
+  Acc* tail = reinterpret_cast<Acc*>(buf + L);
+#pragma unroll
+  for (int i = 0; i < SEG; ++i) {
+    const int t = t0 + i;
+    if (t < t1 && t < L - 1) tail[t] = pre[i] + off;
+  }
+  __syncthreads();
+
+  OutT* __restrict__ out = static_cast<OutT*>(a.out);
+  const size_t nqs = static_cast<size_t>(a.ny) * a.nxh;
+  const size_t oq = shifted_q(q, a.ny, a.nxh);
+  const Acc inv_n = static_cast<Acc>(1) / static_cast<Acc>(n);
+  for (int i = tid; i < a.nl; i += kThreads) {
+    const int d = static_cast<int>(a.lags[i]);
+    const Acc head = tail[L - d - 1];       // psum(L - d)
+    const Acc tl = ptot - tail[d - 1];      // psum(L) - psum(d)
+    const Acc c = static_cast<Acc>(buf[d].x) * inv_n;
+    out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>((head + tl - 2 * c) / static_cast<Acc>(L - d));
+  }
+  if (tid == 0) {
+    const Acc mr2 = sr / L, mi2 = si / L;
+    Acc ps, var;
+    if constexpr (kSub) {
+      var = ptot / L - (mr2 * mr2 + mi2 * mi2);
+      Acc fr = static_cast<Acc>(m_r) + mr2, fi = static_cast<Acc>(m_i) + mi2;
+      if (a.S0) {
+        const C z = static_cast<const C*>(a.S0)[static_cast<size_t>(q) * a.s0_stride];
+        fr += static_cast<Acc>(z.x);
+        fi += static_cast<Acc>(z.y);
+      }
+      ps = var + fr * fr + fi * fi;
+    } else {
+      ps = ptot / L;
+      var = ps - (mr2 * mr2 + mi2 * mi2);
+    }
+    out[static_cast<size_t>(a.nl) * nqs + oq] = static_cast<OutT>(ps);
+    out[static_cast<size_t>(a.nl + 1) * nqs + oq] = static_cast<OutT>(var);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage (d)+(e): direct differences per q.
+// ---------------------------------------------------------------------------
+template <typename T, typename OutT, int PREC, bool USE_LDS>
+__global__ void __launch_bounds__(kThreads) k_diff(TimeArgs a, int q_per_xcd, int nq) {
+  using C = cplx_t<T>;
+  using Acc = std::conditional_t<PREC == PREC_FP32, float, double>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* buf = reinterpret_cast<C*>(smem);
+  const int tid = threadIdx.x;
+  const int q = (blockIdx.x & 7) * q_per_xcd + (blockIdx.x >> 3);
+  if (q >= nq) return;
+  const int L = a.L;
+  const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(q) * a.Lp;
+  const C* ser = src;
+  if constexpr (USE_LDS) {
+    for (int t = tid; t < L; t += kThreads) buf[t] = src[t];
+    __syncthreads();
+    ser = buf;
+  }
+  Acc sr = 0, si = 0, s2 = 0;
+  for (int t = tid; t < L; t += kThreads) {
+    const C f = ser[t];
+    sr += static_cast<Acc>(f.x);
+    si += static_cast<Acc>(f.y);
+    s2 += static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
+  }
+  sr = block_sum(sr);
+  si = block_sum(si);
+  s2 = block_sum(s2);
+
+  OutT* __restrict__ out = static_cast<OutT*>(a.out);
+  const size_t nqs = static_cast<size_t>(a.ny) * a.nxh;
+  const size_t oq = shifted_q(q, a.ny, a.nxh);
+  const int lane = tid & 63, wid = tid >> 6;
+  for (int i = wid; i < a.nl; i += kWaves) {
+    const int d = static_cast<int>(a.lags[i]);
+    Acc acc = 0;
+    for (int t = lane; t + d < L; t += 64) {
+      const C u = ser[t + d], v = ser[t];
+      const Acc dr = static_cast<Acc>(u.x) - static_cast<Acc>(v.x);
+      const Acc di = static_cast<Acc>(u.y) - static_cast<Acc>(v.y);
+      acc += dr * dr + di * di;
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(acc / static_cast<Acc>(L - d));
+  }
+  if (tid == 0) {
+    const Acc mr = sr / L, mi = si / L;
+    Acc ps, var;
+    if (a.S0) {
+      // spectra hold F(t) - F(0): VAR is shift invariant, PS adds |F(0) + mean|^2
+      var = s2 / L - (mr * mr + mi * mi);
+      const C z = static_cast<const C*>(a.S0)[static_cast<size_t>(q) * a.s0_stride];
+      const Acc fr = mr + static_cast<Acc>(z.x), fi = mi + static_cast<Acc>(z.y);
+      ps = var + fr * fr + fi * fi;
+    } else {
+      ps = s2 / L;
+      var = ps - mr * mr - mi * mi;
+    }
+    out[static_cast<size_t>(a.nl) * nqs + oq] = static_cast<OutT>(ps);
+    out[static_cast<size_t>(a.nl + 1) * nqs + oq] = static_cast<OutT>(var);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FFT engine test kernel: one forward transform per workgroup, in place.
+// ---------------------------------------------------------------------------
+template <typename T, int E>
+__global__ void __launch_bounds__(kThreads) k_fft_test(cplx_t<T>* data, FftDesc d, const cplx_t<T>* tw) {
+  using C = cplx_t<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* buf = reinterpret_cast<C*>(smem);
+  C* p = data + static_cast<size_t>(blockIdx.x) * d.n;
+  for (int t = threadIdx.x; t < d.n; t += kThreads) buf[t] = p[t];
+  __syncthreads();
+  fft_lds<T, E, false>(buf, d, 1, d.n, 1, tw, threadIdx.x, kThreads);
+  for (int t = threadIdx.x; t < d.n; t += kThreads) p[t] = buf[t];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename K, typename... Args>
+hipError_t launch(K kernel, unsigned grid, size_t lds, hipStream_t s, Args... args) {
+  if (grid == 0) return hipSuccess;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), lds, s, args...);
+  return hipGetLastError();
+}
+
+template <typename Tin, typename T>
+hipError_t rows_e(const RowArgs& a, unsigned grid, size_t lds, int e, hipStream_t s) {
+  switch (e) {
+    case 16: return launch(k_rows<Tin, T, 16>, grid, lds, s, a);
+    case 32: return launch(k_rows<Tin, T, 32>, grid, lds, s, a);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t rows_t(const RowArgs& a, unsigned grid, size_t lds, int e, hipStream_t s) {
+  switch (a.in_dtype) {
+    case IN_U8: return rows_e<uint8_t, T>(a, grid, lds, e, s);
+    case IN_I16: return rows_e<int16_t, T>(a, grid, lds, e, s);
+    case IN_U16: return rows_e<uint16_t, T>(a, grid, lds, e, s);
+    case IN_I32: return rows_e<int32_t, T>(a, grid, lds, e, s);
+    case IN_U32: return rows_e<uint32_t, T>(a, grid, lds, e, s);
+    case IN_I64: return rows_e<int64_t, T>(a, grid, lds, e, s);
+    case IN_U64: return rows_e<uint64_t, T>(a, grid, lds, e, s);
+    case IN_F32: return rows_e<float, T>(a, grid, lds, e, s);
+    case IN_F64: return rows_e<double, T>(a, grid, lds, e, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int rc_bucket(long long work) {
+  const int e = fft_bucket(work);
+  return e <= 16 ? 16 : e;
+}
+
+}  // namespace
+
+hipError_t launch_rows(const RowArgs& a, int precision, hipStream_t s) {
+  const uint64_t nblocks = (a.frames + a.Tb - 1) / a.Tb;
+  const unsigned grid = static_cast<unsigned>(nblocks * a.H);
+  const int e = rc_bucket(static_cast<long long>(a.G) * a.nx);
+  if (precision == PREC_FP64) {
+    return rows_t<double>(a, grid, sizeof(double2) * a.G * (a.nx + 1), e, s);
+  }
+  return rows_t<float>(a, grid, sizeof(float2) * a.G * (a.nx + 1), e, s);
+}
+
+hipError_t launch_cols(const ColArgs& a, int precision, hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>(a.nblocks * a.nxh);
+  const int e = rc_bucket(static_cast<long long>(a.G) * a.ny);
+  if (precision == PREC_FP64) {
+    const size_t lds = sizeof(double2) * a.ny * a.Tb;
+    if (e == 16) return launch(k_cols<double, 16>, grid, lds, s, a);
+    if (e == 32) return launch(k_cols<double, 32>, grid, lds, s, a);
+  } else {
+    const size_t lds = sizeof(float2) * a.ny * a.Tb;
+    if (e == 16) return launch(k_cols<float, 16>, grid, lds, s, a);
+    if (e == 32) return launch(k_cols<float, 32>, grid, lds, s, a);
+  }
+  return hipErrorInvalidValue;
+}
+
+namespace {
+template <typename T, typename OutT, int PREC>
+hipError_t time_e(const TimeArgs& a, int e, hipStream_t s) {
+  const int nq = a.ny * a.nxh;
+  const int qpx = (nq + 7) / 8;
+  const unsigned grid = static_cast<unsigned>(qpx) * 8u;
+  const size_t lds = sizeof(cplx_t<T>) * a.dt.n;
+  switch (e) {
+    case 4: return launch(k_time<T, OutT, 4, PREC>, grid, lds, s, a, qpx, nq);
+    case 8: return launch(k_time<T, OutT, 8, PREC>, grid, lds, s, a, qpx, nq);
+    case 16: return launch(k_time<T, OutT, 16, PREC>, grid, lds, s, a, qpx, nq);
+    case 32: return launch(k_time<T, OutT, 32, PREC>, grid, lds, s, a, qpx, nq);
+    case 64:
+      if constexpr (sizeof(T) == 4) return launch(k_time<T, OutT, 64, PREC>, grid, lds, s, a, qpx, nq);
+      return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T, typename OutT, int PREC>
+hipError_t diff_l(const TimeArgs& a, hipStream_t s) {
+  const int nq = a.ny * a.nxh;
+  const int qpx = (nq + 7) / 8;
+  const unsigned grid = static_cast<unsigned>(qpx) * 8u;
+  const size_t lds = sizeof(cplx_t<T>) * a.L;
+  if (lds <= 64 * 1024) return launch(k_diff<T, OutT, PREC, true>, grid, lds, s, a, qpx, nq);
+  return launch(k_diff<T, OutT, PREC, false>, grid, 0, s, a, qpx, nq);
+}
+}  // namespace
+
+hipError_t launch_time(const TimeArgs& a, int precision, int out_dtype, hipStream_t s) {
+  const int e = fft_bucket(a.dt.n);
+  const bool f32 = out_dtype == OUT_F32;
+  switch (precision) {
+    case PREC_FP64:
+      return f32 ? time_e<double, float, PREC_FP64>(a, e, s) : time_e<double, double, PREC_FP64>(a, e, s);
+    case PREC_FP32:
+      return f32 ? time_e<float, float, PREC_FP32>(a, e, s) : time_e<float, double, PREC_FP32>(a, e, s);
+    case PREC_FP32C:
+      return f32 ? time_e<float, float, PREC_FP32C>(a, e, s) : time_e<float, double, PREC_FP32C>(a, e, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_diff(const TimeArgs& a, int precision, int out_dtype, hipStream_t s) {
+  const bool f32 = out_dtype == OUT_F32;
+  switch (precision) {
+    case PREC_FP64: return f32 ? diff_l<double, float, PREC_FP64>(a, s) : diff_l<double, double, PREC_FP64>(a, s);
+    case PREC_FP32: return f32 ? diff_l<float, float, PREC_FP32>(a, s) : diff_l<float, double, PREC_FP32>(a, s);
+    case PREC_FP32C: return f32 ? diff_l<float, float, PREC_FP32C>(a, s) : diff_l<float, double, PREC_FP32C>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_fft_test(void* data, int n, int batch, const FftDesc& d, const void* tw, int precision,
+                           hipStream_t s) {
+  const int e = fft_bucket(n);
+  if (precision == PREC_FP64) {
+    using C = double2;
+    const size_t lds = sizeof(C) * n;
+    C* p = static_cast<C*>(data);
+    const C* t = static_cast<const C*>(tw);
+    switch (e) {
+      case 4: return launch(k_fft_test<double, 4>, batch, lds, s, p, d, t);
+      case 8: return launch(k_fft_test<double, 8>, batch, lds, s, p, d, t);
+      case 16: return launch(k_fft_test<double, 16>, batch, lds, s, p, d, t);
+      case 32: return launch(k_fft_test<double, 32>, batch, lds, s, p, d, t);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  using C = float2;
+  const size_t lds = sizeof(C) * n;
+  C* p = static_cast<C*>(data);
+  const C* t = static_cast<const C*>(tw);
+  switch (e) {
+    case 4: return launch(k_fft_test<float, 4>, batch, lds, s, p, d, t);
+    case 8: return launch(k_fft_test<float, 8>, batch, lds, s, p, d, t);
+    case 16: return launch(k_fft_test<float, 16>, batch, lds, s, p, d, t);
+    case 32: return launch(k_fft_test<float, 32>, batch, lds, s, p, d, t);
+    case 64: return launch(k_fft_test<float, 64>, batch, lds, s, p, d, t);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace fddm

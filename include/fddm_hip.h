@@ -1,0 +1,139 @@
+/*
+ * fastddm_amd -- C ABI of the MI355X (gfx950) image structure function engine
+ * (libfddm_hip.so).
+ *
+ * Every entry point returns 0 on success and a non-zero FDDM_ERR_* code on
+ * failure, with a NUL-terminated message written to err[0..errlen) (err may be
+ * NULL).  No entry point exits or aborts the process.  Device memory handed to
+ * the library is never freed by it; host memory is only read (inputs) or
+ * written (outputs).  Calls are synchronous unless a stream is passed.
+ *
+ * Reference interfaces replaced (paths in somexlab/fastddm):
+ *   - ddm_fft_cuda<T> / ddm_diff_cuda<T>   src/cuda/ddm_cuda.h:37-58,
+ *     bound as _core_cuda.ddm_fft_cuda / ddm_diff_cuda in
+ *     src/python/bindings/ddm_cuda_bindings.h:36-55 (nine input dtypes);
+ *   - device utilities get_num_devices / set_device / get_device /
+ *     get_free_device_memory  src/python/bindings/ddm_cuda_bindings.h:26-31,
+ *     src/cuda/gpu_utils.cu:20-116.
+ */
+#ifndef FDDM_HIP_H
+#define FDDM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Input pixel types: the nine overloads of ddm_cuda_bindings.h:36-55. */
+enum {
+    FDDM_U8 = 0,
+    FDDM_I16 = 1,
+    FDDM_U16 = 2,
+    FDDM_I32 = 3,
+    FDDM_U32 = 4,
+    FDDM_I64 = 5,
+    FDDM_U64 = 6,
+    FDDM_F32 = 7,
+    FDDM_F64 = 8
+};
+
+/* Compute precision.  FP64: all stages in double (the reference GPU core
+ * computes in double, src/cuda/memchk_gpu.cu:156,290,397).  FP32: float
+ * spectra, FFTs and sums.  FP32C: float spectra and FFTs, frame-0 and
+ * temporal-mean subtraction, double prefix sums and combination. */
+enum { FDDM_PREC_FP64 = 0, FDDM_PREC_FP32 = 1, FDDM_PREC_FP32C = 2 };
+
+/* Output element type: the reference's Scalar / DTYPE (float64 by default,
+ * float32 with SINGLE_PRECISION; src/python/_config.py.in:16). */
+enum { FDDM_OUT_F64 = 0, FDDM_OUT_F32 = 1 };
+
+enum { FDDM_MODE_DIFF = 0, FDDM_MODE_FFT = 1 };
+
+enum {
+    FDDM_OK = 0,
+    FDDM_ERR_ARG = 1,      /* invalid argument */
+    FDDM_ERR_SIZE = 2,     /* size outside what the kernels support */
+    FDDM_ERR_MEMORY = 3,   /* not enough device memory */
+    FDDM_ERR_HIP = 4,      /* HIP runtime error */
+    FDDM_ERR_DEVICE = 5    /* bad or missing device */
+};
+
+/* ---- device utilities (ddm_cuda_bindings.h:26-31; gpu_utils.cu:20-116) ---- */
+int fddm_get_num_devices(int *count, char *err, size_t errlen);
+/* Out-of-range ids fail like set_device (gpu_utils.cu:40-60). */
+int fddm_set_device(int gpu_id, char *err, size_t errlen);
+int fddm_get_device(int *gpu_id, char *err, size_t errlen);
+/* Free device memory in bytes (get_free_device_memory, gpu_utils.cu:83-116). */
+int fddm_get_free_device_memory(uint64_t *free_bytes, char *err, size_t errlen);
+
+/* ---- whole path, host buffers in and out ----
+ * Replaces ddm_fft_cuda<T> (ddm_cuda.h:52-58, ddm_cuda.cc:181-285) and
+ * ddm_diff_cuda<T> (ddm_cuda.h:37-43, ddm_cuda.cc:25-127).
+ *   img      (length, height, width) C-contiguous pixels of type `dtype`
+ *   lags     num_lags sorted, unique values in [1, length-1]
+ *   nx, ny   padded FFT sizes (>= width, >= height; 7-smooth)
+ *   nt       fft mode only: time FFT length (>= 2*length-1, 7-smooth)
+ *   window   NULL or height*width values of type `window_f32 ? float : double`
+ *   out      (num_lags+2, ny, nx/2+1) of `out_dtype`: D slices, power
+ *            spectrum, variance, fftshifted along y (helper_ddm.cc:41-103).
+ * Uses the current device (fddm_set_device). */
+int fddm_ddm_fft(const void *img, int dtype, uint64_t length, uint64_t height, uint64_t width,
+                 const uint32_t *lags, uint64_t num_lags, uint64_t nx, uint64_t ny, uint64_t nt,
+                 const void *window, int window_f32, int precision, int out_dtype, void *out,
+                 char *err, size_t errlen);
+int fddm_ddm_diff(const void *img, int dtype, uint64_t length, uint64_t height, uint64_t width,
+                  const uint32_t *lags, uint64_t num_lags, uint64_t nx, uint64_t ny,
+                  const void *window, int window_f32, int precision, int out_dtype, void *out,
+                  char *err, size_t errlen);
+/* Drops the device buffers cached by fddm_ddm_fft / fddm_ddm_diff. */
+int fddm_release_cache(char *err, size_t errlen);
+
+/* ---- plans: device-resident inputs and outputs ----
+ * The phases of ddm_fft_cuda / ddm_diff_cuda (compute_fft2,
+ * structure_function_fft/diff, make_shift; ddm_cuda.cu:40-1007) as one
+ * reusable plan whose execute runs all stages on one stream. */
+typedef struct fddm_plan fddm_plan;
+int fddm_plan_create(fddm_plan **plan, int mode, int dtype, uint64_t length, uint64_t height,
+                     uint64_t width, const uint32_t *lags, uint64_t num_lags, uint64_t nx,
+                     uint64_t ny, uint64_t nt, const void *window, int window_f32, int precision,
+                     int out_dtype, char *err, size_t errlen);
+int fddm_plan_destroy(fddm_plan *plan);
+/* Workspace bytes that fddm_plan_execute needs (spectra and intermediates). */
+int fddm_plan_workspace_size(const fddm_plan *plan, uint64_t *bytes);
+/* Output bytes: (num_lags+2) * ny * (nx/2+1) * sizeof(out element). */
+int fddm_plan_output_size(const fddm_plan *plan, uint64_t *bytes);
+/* d_img, d_out, d_workspace: device pointers; stream: hipStream_t or NULL. */
+int fddm_plan_execute(fddm_plan *plan, const void *d_img, void *d_out, void *d_workspace,
+                      void *stream, char *err, size_t errlen);
+/* Per-stage timing with HIP events on the execute stream: begin reserves
+ * events for up to max_steps executes; end waits for them and writes the
+ * summed milliseconds of the stages {frame-0 spectra, row pass, column pass,
+ * time/diff kernel} to stage_ms[4] and the number of timed executes. */
+int fddm_plan_timing_begin(fddm_plan *plan, int max_steps, char *err, size_t errlen);
+int fddm_plan_timing_end(fddm_plan *plan, double *stage_ms, int *steps, char *err, size_t errlen);
+/* Test entry: spatial stages only; copies the q-major spectra (ny*(nx/2+1),
+ * length) to host as interleaved double complex (unshifted, q = ky*nxh+kx). */
+int fddm_plan_spectra(fddm_plan *plan, const void *d_img, void *d_workspace, double *host_spectra,
+                      char *err, size_t errlen);
+
+/* ---- device memory and streams (so callers need no other GPU runtime) ---- */
+int fddm_malloc(void **ptr, uint64_t bytes, char *err, size_t errlen);
+int fddm_free(void *ptr);
+int fddm_memcpy_htod(void *dst, const void *src, uint64_t bytes, char *err, size_t errlen);
+int fddm_memcpy_dtoh(void *dst, const void *src, uint64_t bytes, char *err, size_t errlen);
+int fddm_stream_create(void **stream, char *err, size_t errlen);
+int fddm_stream_destroy(void *stream);
+int fddm_stream_synchronize(void *stream, char *err, size_t errlen);
+
+/* ---- FFT engine test entry: batch forward FFTs of length n, in place on
+ * host data (interleaved double complex); precision FP64 or FP32. */
+int fddm_test_fft(double *data, uint64_t n, uint64_t batch, int precision, char *err,
+                  size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDDM_HIP_H */

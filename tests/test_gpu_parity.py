@@ -1,0 +1,200 @@
+"""GPU parity: the HIP core (through the C ABI) against the CPU oracles on the
+same seeded inputs.  Tolerances (stated per test):
+
+* fp64: every element within rtol 1e-9 of the FP64 oracle, plus an absolute
+  floor of 1e-12 x max|ref| per slice (cancellation at the smallest values);
+* fp32c: the reference's single-precision bar, np.isclose(rtol=1e-3, atol=0)
+  on every D / PS / VAR element (tests/test_ddm.py:116-157), and median
+  relative error <= 1e-5;
+* fp32 (plain): median relative error <= 1e-4 on D, PS within rtol 1e-4.
+"""
+
+import numpy as np
+import pytest
+
+from fastddm_amd import next_fast_len
+from fastddm_amd.speckle import ar1_speckle
+from oracle import oracle_np as onp
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(x, ref):
+    return onp.relative_error(x, ref)
+
+
+def check_fp64(out, ref):
+    for s in range(ref.shape[0]):
+        floor = 1e-12 * np.max(np.abs(ref[s]))
+        assert np.allclose(out[s], ref[s], rtol=1e-9, atol=floor), (s, np.max(np.abs(out[s] - ref[s])))
+
+
+def check_fp32c(out, ref):
+    assert np.isclose(out, ref, rtol=1e-3, atol=0).all(), np.max(relerr(out, ref))
+    assert np.median(relerr(out[:-2], ref[:-2])) <= 1e-5
+
+
+def check_fp32(out, ref):
+    assert np.median(relerr(out[:-2], ref[:-2])) <= 1e-4
+    assert np.allclose(out[-2], ref[-2], rtol=1e-4, atol=0)
+
+
+CHECK = {"fp64": check_fp64, "fp32c": check_fp32c, "fp32": check_fp32}
+
+
+# ------------------------------------------------------------ device utilities
+def test_device_utilities(hip):
+    assert hip.get_num_devices() >= 1
+    hip.set_device(0)
+    assert hip.get_device() == 0
+    assert hip.get_free_device_memory() > 0
+    with pytest.raises(RuntimeError):
+        hip.set_device(hip.get_num_devices() + 3)
+
+
+# ------------------------------------------------------------- FFT engine
+FFT_SIZES = [1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 16, 25, 27, 32, 49, 64, 96, 125, 128, 150, 200, 243,
+             256, 343, 384, 480, 512, 600, 640, 1000, 1024, 2048, 2401, 4000, 4096, 6000, 8000]
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+def test_fft_engine_matches_numpy(hip, precision, tol):
+    rng = np.random.default_rng(7)
+    for n in FFT_SIZES:
+        if precision == "fp64" and n > 8192:
+            continue
+        x = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+        got = hip.device_fft(x, precision)
+        ref = np.fft.fft(x)
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        assert err < tol * max(1.0, np.log2(max(n, 2)) / 4), (n, err)
+
+
+# ------------------------------------------------------ spatial spectra stage
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32c"])
+@pytest.mark.parametrize("shape,nx,ny,window", [
+    ((21, 30, 40), 40, 30, False),
+    ((18, 35, 21), 21, 35, True),
+    ((9, 30, 40), 42, 32, True),
+])
+def test_spatial_spectra_match_oracle(hip, precision, shape, nx, ny, window):
+    L, H, W = shape
+    img = np.random.default_rng(11).integers(0, 4000, shape).astype(np.uint16)
+    win = np.random.default_rng(12).random((H, W)) if window else None
+    plan = hip.Plan("fft", img.dtype, shape, [1], nx, ny, next_fast_len(2 * L, "hip"), win, precision)
+    d_img = hip.DeviceBuffer.from_array(img)
+    d_ws = hip.DeviceBuffer(plan.workspace_size())
+    got = plan.spectra(d_img, d_ws)  # (nq, L), q = ky * nxh + kx
+    base = img.astype(np.float64)
+    if precision != "fp32":
+        base = base - base[0]  # the compensated paths transform I(t) - I(0)
+    ref = onp.spectra(base, nx, ny, win).reshape(L, -1).T
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err < (1e-13 if precision == "fp64" else 2e-6), err
+
+
+# ------------------------------------------------ whole path against the oracle
+CASES = [
+    # (L, H, W, nx, ny, window, lags)
+    (40, 32, 32, 32, 32, False, "all"),
+    (37, 30, 25, 25, 30, True, "all"),
+    (25, 21, 27, 27, 21, False, [1, 2, 3, 10, 24]),
+    (16, 24, 20, 21, 25, True, [1, 5, 15]),  # zero padding in x and y
+    (64, 20, 48, 48, 20, False, "all"),
+]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c", "fp32"])
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("mode", ["fft", "diff"])
+def test_whole_path_matches_oracle(hip, mode, case, precision):
+    L, H, W, nx, ny, window, lags = case
+    img = ar1_speckle(L, H, W, sigma=1.5, diffusivity=0.3, mean=1000, std=150, seed=L + H)
+    win = np.random.default_rng(3).random((H, W)) if window else None
+    lags = np.arange(1, L) if lags == "all" else np.array(lags)
+    ref = onp.ddm_fft(img, lags, nx, ny, 2 * L, win, subtract_mean=True)
+    if mode == "fft":
+        nt = next_fast_len(2 * L, "hip")
+        out = hip.ddm_fft(img, lags, nx, ny, nt, win, precision=precision)
+    else:
+        out = hip.ddm_diff(img, lags, nx, ny, win, precision=precision)
+    assert out.shape == ref.shape
+    CHECK[precision](out, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64,
+                                   np.uint64, np.float32, np.float64])
+def test_every_input_dtype(hip, dtype):
+    img = np.random.default_rng(4).integers(0, 250, (12, 16, 18)).astype(dtype)
+    lags = np.arange(1, 12)
+    ref = onp.ddm_fft(img, lags, 18, 16, 24, subtract_mean=True)
+    check_fp64(hip.ddm_fft(img, lags, 18, 16, 24, None, precision="fp64"), ref)
+    check_fp64(hip.ddm_diff(img, lags, 18, 16, None, precision="fp64"), ref)
+    check_fp32c(hip.ddm_fft(img, lags, 18, 16, 24, None, precision="fp32c"), ref)
+
+
+def test_float32_output_dtype(hip):
+    img = ar1_speckle(20, 16, 16, seed=9)
+    lags = np.arange(1, 20)
+    ref = onp.ddm_fft(img, lags, 16, 16, 40, subtract_mean=True)
+    out = hip.ddm_fft(img, lags, 16, 16, 40, None, precision="fp32c", out_dtype=np.float32)
+    assert out.dtype == np.float32
+    assert np.isclose(out, ref, rtol=1e-3, atol=0).all()
+
+
+def test_no_lags_gives_power_spectrum_and_variance(hip):
+    img = ar1_speckle(10, 16, 16, seed=2)
+    ref = onp.ddm_diff(img, [], 16, 16)
+    for mode in ("fft", "diff"):
+        out = (hip.ddm_fft(img, [], 16, 16, 20, None) if mode == "fft"
+               else hip.ddm_diff(img, [], 16, 16, None))
+        assert out.shape == (2, 16, 9)
+        check_fp64(out, ref)
+
+
+def test_two_frames(hip):
+    img = ar1_speckle(2, 12, 10, seed=5)
+    ref = onp.ddm_diff(img, [1], 10, 12)
+    check_fp64(hip.ddm_fft(img, [1], 10, 12, 4, None), ref)
+    check_fp64(hip.ddm_diff(img, [1], 10, 12, None), ref)
+
+
+def test_reference_cross_core_on_confocal(hip, confocal):
+    """tests/test_ddm.py:53-94 of the reference: every core equals the cpp fft
+    core with np.isclose defaults on confocal[:20], lags 1..9 -- here against
+    the FP64 oracle, for the HIP fft and diff modes in fp64."""
+    lags = np.arange(1, 10)
+    ref = onp.ddm_fft(confocal, lags, 256, 256, 40)
+    nt = next_fast_len(40, "hip")
+    assert np.isclose(hip.ddm_fft(confocal, lags, 256, 256, nt, None), ref).all()
+    assert np.isclose(hip.ddm_diff(confocal, lags, 256, 256, None), ref).all()
+    # single-precision bar of the reference (rtol 1e-3, atol 0)
+    assert np.isclose(hip.ddm_fft(confocal, lags, 256, 256, nt, None, precision="fp32c"), ref,
+                      rtol=1e-3, atol=0).all()
+
+
+# ------------------------------------------------------------ ddm() front end
+def test_ddm_front_end_with_hip_core(hip):
+    import fastddm_amd as fddm
+
+    img = ar1_speckle(30, 24, 20, seed=8)
+    win = fddm.window.blackman_harris(img.shape)
+    lags = [0, 3, 1, 3, 29, 7]
+    with pytest.warns(UserWarning):
+        res = fddm.ddm(img, lags, core="hip", mode="fft", window=win)
+    ref = onp.ddm_fft(img, [1, 3, 7, 29], 20, 24, 60, win, subtract_mean=True)
+    check_fp64(res._data, ref)
+    assert np.array_equal(res.tau, [1, 3, 7, 29])
+    assert res.shape == (4, 24, 11)
+    res_d = fddm.ddm(img, [1, 3, 7, 29], core="hip", mode="diff", window=win, precision="fp32c")
+    check_fp32c(res_d._data, ref)
+    with pytest.raises(RuntimeError):
+        fddm.ddm(img, [1], core="hip", precision="fp16")
+    with pytest.raises(RuntimeError):
+        fddm.ddm(img, [1], core="hip", gpu_id=99)
+
+
+def test_too_long_series_is_rejected(hip):
+    img = np.zeros((5000, 4, 4), np.uint8)
+    with pytest.raises(RuntimeError, match="too long"):
+        hip.ddm_fft(img, [1], 4, 4, 10000, None, precision="fp64")
