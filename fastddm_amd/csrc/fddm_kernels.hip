@@ -23,6 +23,7 @@ namespace fddm {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
+constexpr int kEdge = 32;  // lags with at most this many pairs use direct sums
 
 int fft_bucket(long long work) {
   const long long e = (work + kThreads - 1) / kThreads;
@@ -266,6 +267,17 @@ __global__ void __launch_bounds__(kThreads) k_time(TimeArgs a, int q_per_xcd, in
   const Acc off = block_excl_scan(run, ptot);
   sr = block_sum(sr);
   si = block_sum(si);
+  // Lags with few pairs (L - dt <= kEdge) are summed directly from the first
+  // and last kEdge samples: there the correlation form cancels badly (D is a
+  // sum of a handful of terms, the prefix sums are not).
+  __shared__ C edge[2 * kEdge];
+  if constexpr (kSub) {
+    for (int u = tid; u < kEdge; u += kThreads) {
+      edge[u] = u < L ? buf[u] : C{0, 0};
+      const int s = L - kEdge + u;
+      edge[kEdge + u] = s >= 0 ? buf[s] : C{0, 0};
+    }
+  }
 
   const C* __restrict__ tw = static_cast<const C*>(a.twt);
   fft_lds<T, E, false>(buf, a.dt, 1, n, 1, tw, tid, kThreads);
@@ -291,10 +303,26 @@ __global__ void __launch_bounds__(kThreads) k_time(TimeArgs a, int q_per_xcd, in
   const Acc inv_n = static_cast<Acc>(1) / static_cast<Acc>(n);
   for (int i = tid; i < a.nl; i += kThreads) {
     const int d = static_cast<int>(a.lags[i]);
-    const Acc head = tail[L - d - 1];       // psum(L - d)
-    const Acc tl = ptot - tail[d - 1];      // psum(L) - psum(d)
-    const Acc c = static_cast<Acc>(buf[d].x) * inv_n;
-    out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>((head + tl - 2 * c) / static_cast<Acc>(L - d));
+    Acc D;
+    if (kSub && L - d <= kEdge) {
+      // samples s < kEdge are in edge[s], s >= L - kEdge in edge[s - L + 2 kEdge]
+      Acc acc = 0;
+      for (int t = 0; t < L - d; ++t) {
+        const int s = t + d;
+        const C u = s < kEdge ? edge[s] : edge[s - L + 2 * kEdge];
+        const C v = edge[t];
+        const Acc dr = static_cast<Acc>(u.x) - static_cast<Acc>(v.x);
+        const Acc di = static_cast<Acc>(u.y) - static_cast<Acc>(v.y);
+        acc += dr * dr + di * di;
+      }
+      D = acc / static_cast<Acc>(L - d);
+    } else {
+      const Acc head = tail[L - d - 1];   // psum(L - d)
+      const Acc tl = ptot - tail[d - 1];  // psum(L) - psum(d)
+      const Acc c = static_cast<Acc>(buf[d].x) * inv_n;
+      D = (head + tl - 2 * c) / static_cast<Acc>(L - d);
+    }
+    out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(D);
   }
   if (tid == 0) {
     const Acc mr2 = sr / L, mi2 = si / L;
