@@ -1,0 +1,230 @@
+"""Benchmark of the fft-mode image structure function on one MI355X.
+
+Workload (default): 2000 frames of 512x512 uint16 AR(1) speckle (sigma 1.5 px,
+D 0.2 px^2/frame, mean 1000, std 150, Poisson noise, seed 2), lags 1..1999,
+Blackman-Harris window, time FFT length 4000, float64 output.  One step is one
+whole analysis with the raw stack already in device memory:
+frame-0 spectra -> row pass -> column pass -> time kernel, output in device
+memory.  Reported:
+
+* value: frames/s of the fp32c path (L / seconds per step), the FP64 GPU path
+  timed beside it, the ratio, and the fp32c error against the FP64 GPU result;
+* per-stage times from HIP events on the execute stream, and the roofline of
+  the dominant kernel from its algorithmic bytes per launch;
+* cpu_baseline: the FP64 C restatement of the reference C++ core
+  (oracle/ddm_port.c, single thread) on a bounded sample of the same workload,
+  scaled to the full job.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--skip-cpu]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(L, H, W, b_in, nq, nl, c, o, window):
+    """Compulsory bytes per stage: inputs read once, outputs written once."""
+    spectra = L * nq * c
+    out = (nl + 2) * nq * o
+    spatial = L * H * W * b_in + spectra + (H * W * (c // 2) if window else 0)
+    time_stage = spectra + out
+    return {"spatial": spatial, "time": time_stage, "whole": spatial + time_stage}
+
+
+def run_gpu(core, img, lags, nx, ny, nt, win, precision, steps, warmup, keep_output):
+    L, H, W = img.shape
+    plan = core.Plan("fft", img.dtype, img.shape, lags, nx, ny, nt, win, precision, np.float64)
+    d_img = core.DeviceBuffer.from_array(img)
+    d_out = core.DeviceBuffer(plan.output_size())
+    d_ws = core.DeviceBuffer(plan.workspace_size())
+    stream = core.Stream()
+    for _ in range(warmup):
+        plan.execute(d_img, d_out, d_ws, stream)
+    stream.synchronize()
+    plan.timing_begin(steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute(d_img, d_out, d_ws, stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    stage_ms, n = plan.timing_end()
+    stage_avg = {k: v / max(n, 1) for k, v in stage_ms.items()}
+    out = d_out.to_array(plan.out_shape, np.float64) if keep_output else None
+    for b in (d_img, d_out, d_ws):
+        b.free()
+    stream.destroy()
+    plan.destroy()
+    return wall / steps * 1e3, stage_avg, out
+
+
+def cpu_baseline(img, lags, nx, ny, nt, win, full_L, nq):
+    """Single-thread C restatement on a sample, scaled to the full job."""
+    from oracle import port
+
+    if not os.path.isfile(port._LIB_PATH):
+        port.build()
+    L, H, W = img.shape
+    # spatial stage: 100 of the frames at full size
+    fs = min(100, L)
+    t0 = time.perf_counter()
+    port.rfft2(img[:fs], nx, ny, win, threads=1)
+    t_spatial = (time.perf_counter() - t0) / fs * full_L
+    # time stage: all frames, a 16-row crop (16 * (nx/2+1) q-series)
+    crop = np.ascontiguousarray(img[:, :16, :])
+    wcrop = None if win is None else np.ascontiguousarray(win[:16, :])
+    nq_crop = 16 * (nx // 2 + 1)
+    t0 = time.perf_counter()
+    port.rfft2(crop, nx, 16, wcrop, threads=1)
+    t_crop_spatial = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    port.ddm_fft(crop, lags, nx, 16, nt, wcrop, chunk_size=257, threads=1)
+    t_crop_total = time.perf_counter() - t0
+    t_time = max(t_crop_total - t_crop_spatial, 0.0) / nq_crop * nq
+    total = t_spatial + t_time
+    return {
+        "value": full_L / total,
+        "unit": "frames/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"spatial stage timed on {fs} of {full_L} frames (512x512), time stage on all "
+                   f"{full_L} frames for {nq_crop} of {nq} q-series (16-row crop); both scaled "
+                   f"linearly to the full job; est. {total:.1f} s per analysis"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=2000)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(whole stacks would be routed to GPUs independently). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from fastddm_amd import _core_hip as core
+    from fastddm_amd import next_fast_len
+    from fastddm_amd.speckle import ar1_speckle
+    from fastddm_amd.window import blackman_harris
+
+    if core.get_num_devices() < 1:
+        raise RuntimeError("no AMD GPU visible")
+    core.set_device(0)
+
+    L, S = args.frames, args.size
+    t0 = time.perf_counter()
+    img = ar1_speckle(L, S, S, sigma=1.5, diffusivity=0.2, mean=1000.0, std=150.0,
+                      dtype=np.uint16, seed=2)
+    log(f"generated {img.shape} {img.dtype} speckle in {time.perf_counter() - t0:.1f} s")
+    win = blackman_harris((S, S))
+    lags = np.arange(1, L)
+    nx = next_fast_len(S, "hip")
+    ny = nx
+    nt = next_fast_len(2 * L, "hip")
+    nxh = nx // 2 + 1
+    nq = ny * nxh
+
+    ms32, st32, out32 = run_gpu(core, img, lags, nx, ny, nt, win, "fp32c", args.steps, args.warmup,
+                                keep_output=not args.skip_fp64)
+    log(f"fp32c: {ms32:.3f} ms/step, stages {st32}")
+    fp64 = None
+    if not args.skip_fp64:
+        ms64, st64, out64 = run_gpu(core, img, lags, nx, ny, nt, win, "fp64", args.steps,
+                                    args.warmup, keep_output=True)
+        log(f"fp64: {ms64:.3f} ms/step, stages {st64}")
+        d32, d64 = out32[:-2].ravel(), out64[:-2].ravel()
+        m = d64 != 0
+        rel = np.abs(d32[m] - d64[m]) / np.abs(d64[m])
+        fp64 = {
+            "frames_per_s": L / (ms64 / 1e3),
+            "ms_per_step": ms64,
+            "stages_ms": st64,
+            "fp32c_vs_fp64_gpu_rel_err": {"median": float(np.median(rel)), "max": float(rel.max())},
+        }
+        del out32, out64, d32, d64, rel
+
+    b = algorithmic_bytes(L, S, S, img.itemsize, nq, len(lags), 8, 8, True)
+    kernels = {
+        "time_wk": (st32["time_or_diff"], b["time"]),
+        "spatial_rows_cols": (st32["row_pass"] + st32["column_pass"], b["spatial"]),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    dom_ms, dom_bytes = kernels[dom]
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.isfile(tf):
+        with open(tf) as f:
+            t = json.load(f)
+        entry = t.get("k_time_fp32c_512x2000") if dom == "time_wk" else None
+        if entry:
+            traffic = entry.get("bytes_per_launch")
+
+    result = {
+        "metric": "fft-mode frames/s + HBM roofline %, 512^2x2000 fp32 vs fp64; serve p50/p99",
+        "value": L / (ms32 / 1e3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic AR(1) Brownian speckle, uint16, seed 2 (device-resident input)",
+        "config": {
+            "workload": (f"fft mode, {L} frames of {S}x{S} uint16, lags 1..{L - 1}, Blackman-Harris "
+                         f"window, nt={nt}, precision fp32c, float64 output"),
+            "frames": L, "height": S, "width": S, "nt": nt, "lags": len(lags),
+            "precision": "fp32c", "parallelism": "single GPU",
+        },
+        "stages_ms": st32,
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes": dom_bytes,
+        },
+        "whole_path": {
+            "algorithmic_bytes": b["whole"],
+            "effective_GBs": b["whole"] / (ms32 / 1e3) / 1e9,
+            "frac_of_hbm_peak": b["whole"] / (ms32 / 1e3) / 1e9 / HBM_PEAK_GBS,
+        },
+    }
+    if fp64 is not None:
+        result["fp64"] = fp64
+        result["speedup_vs_fp64"] = fp64["ms_per_step"] / ms32
+    if not args.skip_cpu:
+        t0 = time.perf_counter()
+        result["cpu_baseline"] = cpu_baseline(img, lags, nx, ny, nt, win, L, nq)
+        log(f"cpu baseline sample took {time.perf_counter() - t0:.1f} s")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

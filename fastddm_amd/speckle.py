@@ -7,9 +7,14 @@ so the field decorrelates like Brownian motion with diffusivity D
 (px^2/frame).  Every frame is real(ifft2(amplitude)).  The stack is then
 mapped affinely to the target mean / std (statistics taken over the whole
 stack), optionally Poisson-sampled, clipped to the dtype range and cast.
+
+Randomness: ``SeedSequence(seed)`` spawns one child stream for the initial
+amplitude and one per frame (innovation, then shot noise), so the stack is a
+pure function of the seed whatever the number of worker threads.
 """
 
 import os
+from concurrent.futures import ThreadPoolExecutor
 from typing import Optional
 
 import numpy as np
@@ -18,9 +23,10 @@ from scipy import fft as sfft
 
 def _workers() -> int:
     try:
-        return max(1, min(16, len(os.sched_getaffinity(0))))
+        n = len(os.sched_getaffinity(0))
     except AttributeError:
-        return max(1, min(16, os.cpu_count() or 1))
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def ar1_speckle(
@@ -38,44 +44,60 @@ def ar1_speckle(
     clip: Optional[tuple] = None,
 ) -> np.ndarray:
     """Return an (length, height, width) stack of `dtype` (see module doc)."""
-    rng = np.random.default_rng(seed)
+    kids = np.random.SeedSequence(seed).spawn(length + 1)
     ky = 2.0 * np.pi * np.fft.fftfreq(height)
     kx = 2.0 * np.pi * np.fft.fftfreq(width)
     q2 = ky[:, None] ** 2 + kx[None, :] ** 2
-    env = np.exp(-0.5 * q2 * sigma**2)
-    rho = np.exp(-diffusivity * q2)
-    innov = np.sqrt(np.maximum(0.0, 1.0 - rho**2)) * env
+    env = np.exp(-0.5 * q2 * sigma**2).astype(np.float32)
+    rho = np.exp(-diffusivity * q2).astype(np.float32)
+    innov = (np.sqrt(np.maximum(0.0, 1.0 - np.exp(-diffusivity * q2) ** 2)) * env).astype(np.float32)
+    shape = (height, width)
+    scale = np.float32(1.0 / np.sqrt(2.0))
 
-    def noise():
-        return (rng.standard_normal((height, width)) + 1j * rng.standard_normal((height, width))) / np.sqrt(2.0)
+    def cnoise(rng):
+        re = rng.standard_normal(shape, dtype=np.float32)
+        im = rng.standard_normal(shape, dtype=np.float32)
+        return (re + 1j * im).astype(np.complex64) * scale
 
-    amp = noise() * env
-    frames = np.empty((length, height, width), dtype=np.float32)
     workers = _workers()
-    for t in range(length):
-        frames[t] = sfft.ifft2(amp, workers=workers).real
-        amp = rho * amp + innov * noise()
-    m = float(frames.mean(dtype=np.float64))
-    s = float(frames.std(dtype=np.float64))
-    scale = std / s if s > 0 else 0.0
-    frames -= m
-    frames *= scale
-    frames += mean
-    if poisson:
-        np.clip(frames, 0.0, None, out=frames)
-        frames = rng.poisson(frames).astype(np.float64)
-    if np.issubdtype(np.dtype(dtype), np.integer):
-        info = np.iinfo(dtype)
-        lo, hi = (clip if clip is not None else (info.min, info.max))
-        np.clip(frames, lo, hi, out=frames)
-        return np.rint(frames).astype(dtype)
-    if clip is not None:
-        np.clip(frames, clip[0], clip[1], out=frames)
-    return frames.astype(dtype)
+    frames = np.empty((length, height, width), dtype=np.float32)
+    amp = cnoise(np.random.default_rng(kids[0])) * env
+    chunk = 64
+    with ThreadPoolExecutor(workers) as ex:
+        for c0 in range(0, length, chunk):
+            c1 = min(length, c0 + chunk)
+            eps = list(ex.map(lambda k: cnoise(np.random.default_rng(k)), kids[c0 + 1 : c1 + 1]))
+            amps = np.empty((c1 - c0, height, width), dtype=np.complex64)
+            for i in range(c1 - c0):
+                amps[i] = amp
+                amp = rho * amp + innov * eps[i]
+            frames[c0:c1] = sfft.ifft2(amps, workers=workers).real
+        m = float(frames.mean(dtype=np.float64))
+        s = float(frames.std(dtype=np.float64))
+        a = np.float32(std / s if s > 0 else 0.0)
+        b = np.float32(mean - m * (std / s if s > 0 else 0.0))
+        is_int = np.issubdtype(np.dtype(dtype), np.integer)
+        if is_int:
+            info = np.iinfo(dtype)
+            lo, hi = clip if clip is not None else (info.min, info.max)
+        else:
+            lo, hi = clip if clip is not None else (-np.inf, np.inf)
+        out = np.empty((length, height, width), dtype=dtype)
+
+        def finish(t):
+            x = frames[t] * a + b
+            if poisson:
+                rng = np.random.default_rng(kids[t + 1].spawn(1)[0])
+                x = rng.poisson(np.clip(x, 0.0, None).astype(np.float64)).astype(np.float64)
+            x = np.clip(x, lo, hi)
+            out[t] = np.rint(x) if is_int else x
+
+        list(ex.map(finish, range(length)))
+    return out
 
 
-# The five workloads the benchmarks and tests draw from (sizes, dynamics and
-# intensity statistics of each stack).
+# Workloads the benchmarks and tests draw from (sizes, dynamics and intensity
+# statistics of each stack).
 CONFIGS = {
     "cpu_128": dict(length=64, height=128, width=128, sigma=2.0, diffusivity=0.5,
                     mean=2000.0, std=300.0, dtype=np.uint16, seed=1),
