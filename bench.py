@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 device-to-device copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -130,6 +132,13 @@ def main():
     if core.get_num_devices() < 1:
         raise RuntimeError("no AMD GPU visible")
     core.set_device(0)
+    if args.copy_calibration:
+        n = 1 << 30
+        a, b = core.DeviceBuffer(n), core.DeviceBuffer(n)
+        for _ in range(3):
+            b.copy_from(a, n)
+        log("copied 3 x 1 GiB device to device")
+        return
 
     L, S = args.frames, args.size
     t0 = time.perf_counter()

@@ -68,6 +68,7 @@ def lib() -> ctypes.CDLL:
         "fddm_free": [p],
         "fddm_memcpy_htod": [p, p, u64, cp, sz],
         "fddm_memcpy_dtoh": [p, p, u64, cp, sz],
+        "fddm_memcpy_dtod": [p, p, u64, cp, sz],
         "fddm_stream_create": [ctypes.POINTER(p), cp, sz],
         "fddm_stream_destroy": [p],
         "fddm_stream_synchronize": [p, cp, sz],
@@ -101,6 +102,7 @@ EXPORTED_SYMBOLS = (
     "fddm_free",
     "fddm_memcpy_htod",
     "fddm_memcpy_dtoh",
+    "fddm_memcpy_dtod",
     "fddm_stream_create",
     "fddm_stream_destroy",
     "fddm_stream_synchronize",
@@ -240,6 +242,10 @@ class DeviceBuffer:
             raise RuntimeError("device buffer smaller than requested array")
         _call("fddm_memcpy_dtoh", out.ctypes.data, self.ptr, out.nbytes)
         return out
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int) -> None:
+        """Device-to-device copy of nbytes from `other` into this buffer."""
+        _call("fddm_memcpy_dtod", self.ptr, other.ptr, int(nbytes))
 
     def free(self) -> None:
         if self.ptr is not None and self.ptr.value:

@@ -27,6 +27,7 @@ struct fddm_plan {
   int out_dtype = OUT_F64;
   uint64_t L = 0, H = 0, W = 0, nx = 0, ny = 0, nt = 0, nxh = 0, nq = 0, nl = 0;
   int Tb = 2, Gx = 1, Gy = 1;
+  int logn_t = 0;  // time transform length 2^logn_t (fft mode)
   uint64_t Lp = 0, nblocks = 0;
   bool subtract0 = false;
   FftDesc dx{}, dy{}, dt{};
@@ -119,16 +120,52 @@ bool factorize(uint64_t n, FftDesc& d) {
   return r == 1;
 }
 
-// w[m] = exp(-2 pi i m / n), evaluated in long double and rounded once.
+// Per-pass twiddle tables in execution order (radix 8s, 4s, 2s, 3s, 5s, 7s):
+// pass with radix R after Ns points stores exp(-2 pi i r k / (Ns R)) at
+// k (R-1) + r - 1 for k < Ns, r = 1 .. R-1; n - 1 entries in all.  Evaluated
+// in long double and rounded once.
 template <typename T>
-std::vector<T> twiddles(uint64_t n) {
-  std::vector<T> w(2 * n);
+std::vector<T> twiddles(const FftDesc& d) {
+  std::vector<T> w;
+  w.reserve(2 * static_cast<size_t>(d.n));
   const long double two_pi = 6.283185307179586476925286766559005768L;
-  for (uint64_t m = 0; m < n; ++m) {
-    const long double ang = -two_pi * static_cast<long double>(m) / static_cast<long double>(n);
-    w[2 * m] = static_cast<T>(cosl(ang));
-    w[2 * m + 1] = static_cast<T>(sinl(ang));
-  }
+  const int counts[6] = {d.c8, d.c4, d.c2, d.c3, d.c5, d.c7};
+  const int radix[6] = {8, 4, 2, 3, 5, 7};
+  long long ns = 1;
+  for (int g = 0; g < 6; ++g)
+    for (int i = 0; i < counts[g]; ++i) {
+      const int R = radix[g];
+      for (long long k = 0; k < ns; ++k)
+        for (int r = 1; r < R; ++r) {
+          const long double ang = -two_pi * static_cast<long double>(r * k) / static_cast<long double>(ns * R);
+          w.push_back(static_cast<T>(cosl(ang)));
+          w.push_back(static_cast<T>(sinl(ang)));
+        }
+      ns *= R;
+    }
+  if (w.empty()) w.assign(2, T(0));
+  return w;
+}
+
+// Twiddles of the power-of-two time plan: FFT1 passes 1, 2 then FFT2 passes
+// 1, 2 (p2_plan radices), each [k][r] = exp(-2 pi i r k / (NS R)).
+template <typename T>
+std::vector<T> p2_twiddles(int logn) {
+  const P2Plan P = p2_plan(logn);
+  std::vector<T> w;
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  auto pass = [&](long long R, long long NS) {
+    for (long long k = 0; k < NS; ++k)
+      for (long long r = 0; r < R; ++r) {
+        const long double ang = -two_pi * static_cast<long double>(r * k) / static_cast<long double>(NS * R);
+        w.push_back(static_cast<T>(cosl(ang)));
+        w.push_back(static_cast<T>(sinl(ang)));
+      }
+  };
+  pass(P.r1, P.r0);
+  pass(P.r2, static_cast<long long>(P.r0) * P.r1);
+  pass(P.r1, P.r2);
+  pass(P.r0, static_cast<long long>(P.r2) * P.r1);
   return w;
 }
 
@@ -138,12 +175,12 @@ int upload(void** dst, const void* src, size_t bytes, char* err, size_t errlen) 
   return FDDM_OK;
 }
 
-int upload_twiddles(void** dst, uint64_t n, bool f64, char* err, size_t errlen) {
+int upload_twiddles(void** dst, const FftDesc& d, bool f64, char* err, size_t errlen) {
   if (f64) {
-    auto w = twiddles<double>(n);
+    auto w = twiddles<double>(d);
     return upload(dst, w.data(), w.size() * sizeof(double), err, errlen);
   }
-  auto w = twiddles<float>(n);
+  auto w = twiddles<float>(d);
   return upload(dst, w.data(), w.size() * sizeof(float), err, errlen);
 }
 
@@ -210,8 +247,8 @@ int plan_create(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H, ui
   // Frames per block: runs of Tb * sizeof(complex) = 128 B in the q-major
   // store, halved while a column slab (ny x Tb complex) exceeds 64 KiB.
   p->Tb = f64 ? 8 : 16;
-  while (p->Tb > 2 && ny * p->Tb * csz > 64 * 1024) p->Tb /= 2;
-  if (ny * p->Tb * csz > kLdsLimit)
+  while (p->Tb > 2 && lds_padded_size(static_cast<int>(ny) * p->Tb) * csz > 64 * 1024) p->Tb /= 2;
+  if (lds_padded_size(static_cast<int>(ny) * p->Tb) * csz > kLdsLimit)
     return fail(set_err(err, errlen, FDDM_ERR_SIZE, "ny=%llu too large for the column pass", (unsigned long long)ny));
   // Transforms handled at once by the row / column kernels (<= 16 or 32
   // elements per thread).
@@ -223,31 +260,39 @@ int plan_create(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H, ui
   if (fft_bucket(static_cast<long long>(p->Gx) * nx) == 0 || fft_bucket(p->Gx * nx) > 32 ||
       fft_bucket(static_cast<long long>(p->Gy) * ny) > 32 || fft_bucket(p->Gy * ny) == 0)
     return fail(set_err(err, errlen, FDDM_ERR_SIZE, "nx or ny above 8192"));
-  if (p->Gx * (nx + 1) * csz > kLdsLimit)
+  if (lds_padded_size(static_cast<int>(p->Gx * (nx + 1))) * csz > kLdsLimit)
     return fail(set_err(err, errlen, FDDM_ERR_SIZE, "nx=%llu too large for the row pass", (unsigned long long)nx));
   p->nblocks = (L + p->Tb - 1) / p->Tb;
   p->Lp = p->nblocks * p->Tb;
 
   if (mode == MODE_FFT) {
+    // The caller's nt (>= 2L - 1) fixes no result: D is the same for every
+    // zero-padded length >= 2L - 1.  The time kernel runs at the power of two
+    // 2^logn_t >= max(2L - 1, 512).
     if (nt < 2 * L - 1) return fail(set_err(err, errlen, FDDM_ERR_ARG, "nt must be >= 2*length-1"));
-    if (!factorize(nt, p->dt))
-      return fail(set_err(err, errlen, FDDM_ERR_SIZE, "nt=%llu must be 2^a 3^b 5^c 7^d", (unsigned long long)nt));
-    const int e = fft_bucket(static_cast<long long>(nt));
-    if (e == 0 || (f64 && e > 32) || nt * csz > kLdsLimit)
+    int logn = kP2MinLog;
+    while ((1ull << logn) < 2 * L - 1) ++logn;
+    const int maxlog = f64 ? kP2MaxLog - 1 : kP2MaxLog;
+    if (logn > maxlog || p2_lds_bytes(logn, precision) > kLdsLimit)
       return fail(set_err(err, errlen, FDDM_ERR_SIZE,
-                          "nt=%llu too long for the LDS-resident time kernel (max %d in %s)",
-                          (unsigned long long)nt, f64 ? 8192 : 16384, f64 ? "fp64" : "fp32"));
-    // k_time keeps ceil(L / 256) prefix sums per thread in a register array
-    // of E / 2 entries.
-    if ((L + 255) / 256 > static_cast<uint64_t>(e / 2))
-      return fail(set_err(err, errlen, FDDM_ERR_SIZE, "length %llu too large for nt=%llu",
-                          (unsigned long long)L, (unsigned long long)nt));
+                          "nt=%llu too long for the LDS-resident time kernel (at most %d frames in %s)",
+                          (unsigned long long)nt, 1 << (maxlog - 1), f64 ? "fp64" : "fp32"));
+    p->logn_t = logn;
   }
 
   int rc;
-  if ((rc = upload_twiddles(&p->d_twx, nx, f64, err, errlen))) return fail(rc);
-  if ((rc = upload_twiddles(&p->d_twy, ny, f64, err, errlen))) return fail(rc);
-  if (mode == MODE_FFT && (rc = upload_twiddles(&p->d_twt, nt, f64, err, errlen))) return fail(rc);
+  if ((rc = upload_twiddles(&p->d_twx, p->dx, f64, err, errlen))) return fail(rc);
+  if ((rc = upload_twiddles(&p->d_twy, p->dy, f64, err, errlen))) return fail(rc);
+  if (mode == MODE_FFT) {
+    if (f64) {
+      auto w = p2_twiddles<double>(p->logn_t);
+      rc = upload(&p->d_twt, w.data(), w.size() * sizeof(double), err, errlen);
+    } else {
+      auto w = p2_twiddles<float>(p->logn_t);
+      rc = upload(&p->d_twt, w.data(), w.size() * sizeof(float), err, errlen);
+    }
+    if (rc) return fail(rc);
+  }
   if ((rc = upload(&p->d_lags, lags, nl * sizeof(uint32_t), err, errlen))) return fail(rc);
   if (window) {
     const size_t npx = H * W;
@@ -357,7 +402,7 @@ int plan_execute(fddm_plan* p, const void* d_img, void* d_out, void* ws, hipStre
   ta.twt = p->d_twt;
   ta.out = d_out;
   if (p->mode == MODE_FFT) {
-    FDDM_HIP(launch_time(ta, p->precision, p->out_dtype, s));
+    FDDM_HIP(launch_time_p2(ta, p->logn_t, p->precision, p->out_dtype, s));
   } else {
     FDDM_HIP(launch_diff(ta, p->precision, p->out_dtype, s));
   }
@@ -610,6 +655,11 @@ int fddm_memcpy_dtoh(void* dst, const void* src, uint64_t bytes, char* err, size
   return FDDM_OK;
 }
 
+int fddm_memcpy_dtod(void* dst, const void* src, uint64_t bytes, char* err, size_t errlen) {
+  FDDM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return FDDM_OK;
+}
+
 int fddm_stream_create(void** stream, char* err, size_t errlen) {
   if (!stream) return set_err(err, errlen, FDDM_ERR_ARG, "NULL stream");
   hipStream_t s = nullptr;
@@ -634,10 +684,11 @@ int fddm_test_fft(double* data, uint64_t n, uint64_t batch, int precision, char*
   const bool f64 = precision == PREC_FP64;
   const size_t csz = f64 ? 16 : 8;
   const int e = fft_bucket(static_cast<long long>(n));
-  if (e == 0 || (f64 && e > 32) || n * csz > kLdsLimit) return set_err(err, errlen, FDDM_ERR_SIZE, "n too large");
+  if (e == 0 || (f64 && e > 32) || lds_padded_size(static_cast<int>(n)) * csz > kLdsLimit)
+    return set_err(err, errlen, FDDM_ERR_SIZE, "n too large");
   if (batch > (1u << 30)) return set_err(err, errlen, FDDM_ERR_SIZE, "batch too large");
   void* tw = nullptr;
-  int rc = upload_twiddles(&tw, n, f64, err, errlen);
+  int rc = upload_twiddles(&tw, d, f64, err, errlen);
   if (rc) return rc;
   void* buf = nullptr;
   hipError_t he = hipMalloc(&buf, n * batch * csz);

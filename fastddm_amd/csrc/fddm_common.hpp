@@ -39,6 +39,27 @@ struct FftDesc {
   int c8, c4, c2, c3, c5, c7;
 };
 
+// Elements of an LDS buffer holding n logical elements with one pad slot
+// every 32 (see lds_pad in fddm_fft.hpp).
+__host__ __device__ constexpr int lds_padded_size(int n) { return n + (n >> 5) + 1; }
+
+// Power-of-two plans of the time kernel: N = 2^logn points, T threads per
+// transform, three passes of radices R0, R1, R2 (forward order; the second
+// transform runs them in reverse).  Every pass has a whole number of
+// butterflies per thread: N / (R * T) >= 1.
+struct P2Plan {
+  int threads, r0, r1, r2;
+};
+__host__ __device__ constexpr P2Plan p2_plan(int logn) {
+  return logn == 9    ? P2Plan{64, 8, 8, 8}
+         : logn == 10 ? P2Plan{64, 16, 16, 4}
+         : logn == 11 ? P2Plan{128, 16, 16, 8}
+         : logn == 12 ? P2Plan{256, 16, 16, 16}
+         : logn == 13 ? P2Plan{256, 16, 16, 32}
+                      : P2Plan{256, 16, 32, 32};
+}
+constexpr int kP2MinLog = 9, kP2MaxLog = 14;
+
 template <typename T> struct Cplx;
 template <> struct Cplx<float> { using type = float2; };
 template <> struct Cplx<double> { using type = double2; };

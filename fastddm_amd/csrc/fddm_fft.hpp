@@ -13,8 +13,8 @@
 // Butterflies: radix 2, 4, 8 in closed form; radix 3, 5, 7 from the
 // symmetric-pair form of the DFT,
 //   X_s = v_0 + sum_r (v_r + v_{R-r}) cos(2 pi r s / R) - i (v_r - v_{R-r}) sin(2 pi r s / R).
-// Twiddles come from a per-length table w_n[m] = exp(-2 pi i m / n) built on
-// the host in extended precision.
+// Twiddles come from per-pass tables exp(-2 pi i r k / (Ns R)) built on the
+// host in extended precision and stored contiguously per butterfly.
 #pragma once
 
 #include "fddm_common.hpp"
@@ -143,15 +143,23 @@ __device__ __forceinline__ void butterfly(cplx_t<T> (&v)[R]) {
   }
 }
 
+// LDS address of logical element e: one pad slot every 32 elements spreads
+// the strided Stockham writes of the early passes over the banks.
+__device__ __forceinline__ int lds_pad(int e) { return e + (e >> 5); }
+
 // One in-place radix-R pass over nb transforms of length n held in LDS.
-// Element idx of transform ti lives at buf[ti * bstride + idx * estride].
-// TIF: consecutive work items walk the transform index first (use when the
-// transforms are interleaved, estride = nb); otherwise the butterfly index.
-// MB work items per thread must cover n / R * nb items.
+// Element idx of transform ti lives at buf[lds_pad(off + ti * bstride + idx * estride)].
+// TIF: consecutive work items walk the transform index first (nb a power of
+// two, lognb = log2 nb; used when the transforms are interleaved); otherwise
+// the butterfly index first.  MB work items per thread must cover n / R * nb.
+// tw: this pass's table, tw[k (R-1) + r - 1] = exp(-2 pi i r k / (ns R)).
+// Item indices advance incrementally (m = n / R is a multiple of ns), so no
+// per-item integer division is needed.
 template <typename T, int R, int MB, bool TIF>
-__device__ __forceinline__ void stockham_pass(cplx_t<T>* buf, int n, int ns, int nb, int bstride,
-                                              int estride, const cplx_t<T>* __restrict__ tw,
-                                              int tid, int nthr) {
+__device__ __forceinline__ void stockham_pass(cplx_t<T>* buf, int off, int n, int ns, int nb,
+                                              int lognb, int bstride, int estride,
+                                              const cplx_t<T>* __restrict__ tw, int tid,
+                                              int nthr) {
   using C = cplx_t<T>;
   // Opaque copy of the thread index: keeps the per-item index arithmetic
   // inside the pass instead of letting it be hoisted (and kept live) across
@@ -159,32 +167,46 @@ __device__ __forceinline__ void stockham_pass(cplx_t<T>* buf, int n, int ns, int
   asm volatile("" : "+v"(tid));
   const int m = n / R;
   const int items = m * nb;
-  const int tws = n / (ns * R);
+  int j, ti, dj, dti;
+  if constexpr (TIF) {
+    ti = tid & (nb - 1);
+    j = tid >> lognb;
+    dj = nthr >> lognb;
+    dti = 0;
+  } else {
+    j = tid % m;
+    ti = tid / m;
+    dj = nthr % m;
+    dti = nthr / m;
+  }
+  int k = j % ns;
+  const int dk = dj % ns;
   C v[MB][R];
   int obase[MB];
 #pragma unroll
   for (int b = 0; b < MB; ++b) {
-    const int w = tid + b * nthr;
     obase[b] = -1;
-    if (w < items) {
-      int j, ti;
-      if constexpr (TIF) {
-        ti = w % nb;
-        j = w / nb;
-      } else {
-        j = w % m;
-        ti = w / m;
-      }
-      const int k = j % ns;
-      const C* src = buf + ti * bstride;
+    if (tid + b * nthr < items) {
+      const int base = off + ti * bstride;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = src[(j + r * m) * estride];
+      for (int r = 0; r < R; ++r) v[b][r] = buf[lds_pad(base + (j + r * m) * estride)];
       if (ns > 1) {
+        const C* __restrict__ t = tw + k * (R - 1);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * k * tws]);
+        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], t[r - 1]);
       }
       butterfly<T, R>(v[b]);
-      obase[b] = ti * bstride + ((j - k) * R + k) * estride;
+      obase[b] = base + (j * R - k * (R - 1)) * estride;
+    }
+    j += dj;
+    k += dk;
+    if (k >= ns) k -= ns;
+    if constexpr (!TIF) {
+      ti += dti;
+      if (j >= m) {
+        j -= m;
+        ++ti;
+      }
     }
   }
   __syncthreads();
@@ -193,34 +215,37 @@ __device__ __forceinline__ void stockham_pass(cplx_t<T>* buf, int n, int ns, int
   for (int b = 0; b < MB; ++b) {
     if (obase[b] >= 0) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[obase[b] + r * ostep] = v[b][r];
+      for (int r = 0; r < R; ++r) buf[lds_pad(obase[b] + r * ostep)] = v[b][r];
     }
   }
   __syncthreads();
 }
 
 // Forward FFT (unnormalised) of nb transforms in LDS.  E bounds the elements
-// per thread: E * nthr >= n * nb.  The caller synchronises before the call;
-// the data are synchronised on return.  One loop per radix (rather than a
-// switch over a pass list) keeps the register allocation of the six pass
-// variants disjoint.
+// per thread: E * nthr >= n * nb.  `tw` is the plan's concatenated per-pass
+// twiddle table (pass order: 8s, 4s, 2s, 3s, 5s, 7s).  The caller
+// synchronises before the call; the data are synchronised on return.  One
+// loop per radix (rather than a switch over a pass list) keeps the register
+// allocation of the six pass variants disjoint.
 template <typename T, int E, bool TIF>
-__device__ void fft_lds(cplx_t<T>* buf, const FftDesc& d, int nb, int bstride, int estride,
-                        const cplx_t<T>* __restrict__ tw, int tid, int nthr) {
+__device__ void fft_lds(cplx_t<T>* buf, int off, const FftDesc& d, int nb, int lognb, int bstride,
+                        int estride, const cplx_t<T>* __restrict__ tw, int tid, int nthr) {
   const int n = d.n;
   int ns = 1;
-  for (int i = 0; i < d.c8; ++i, ns *= 8)
-    stockham_pass<T, 8, (E + 7) / 8, TIF>(buf, n, ns, nb, bstride, estride, tw, tid, nthr);
-  for (int i = 0; i < d.c4; ++i, ns *= 4)
-    stockham_pass<T, 4, (E + 3) / 4, TIF>(buf, n, ns, nb, bstride, estride, tw, tid, nthr);
-  for (int i = 0; i < d.c2; ++i, ns *= 2)
-    stockham_pass<T, 2, (E + 1) / 2, TIF>(buf, n, ns, nb, bstride, estride, tw, tid, nthr);
-  for (int i = 0; i < d.c3; ++i, ns *= 3)
-    stockham_pass<T, 3, (E + 2) / 3, TIF>(buf, n, ns, nb, bstride, estride, tw, tid, nthr);
-  for (int i = 0; i < d.c5; ++i, ns *= 5)
-    stockham_pass<T, 5, (E + 4) / 5, TIF>(buf, n, ns, nb, bstride, estride, tw, tid, nthr);
-  for (int i = 0; i < d.c7; ++i, ns *= 7)
-    stockham_pass<T, 7, (E + 6) / 7, TIF>(buf, n, ns, nb, bstride, estride, tw, tid, nthr);
+#define FDDM_PASSES(RAD, COUNT)                                                              \
+  for (int i = 0; i < d.COUNT; ++i) {                                                        \
+    stockham_pass<T, RAD, (E + RAD - 1) / RAD, TIF>(buf, off, n, ns, nb, lognb, bstride,     \
+                                                    estride, tw, tid, nthr);                 \
+    tw += ns * (RAD - 1);                                                                    \
+    ns *= RAD;                                                                               \
+  }
+  FDDM_PASSES(8, c8)
+  FDDM_PASSES(4, c4)
+  FDDM_PASSES(2, c2)
+  FDDM_PASSES(3, c3)
+  FDDM_PASSES(5, c5)
+  FDDM_PASSES(7, c7)
+#undef FDDM_PASSES
 }
 
 }  // namespace fddm

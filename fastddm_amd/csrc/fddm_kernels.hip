@@ -154,18 +154,18 @@ __global__ void __launch_bounds__(kThreads) k_rows(RowArgs a) {
           vb *= wv;
         }
       }
-      buf[p * bstride + x] = C{va, vb};
+      buf[lds_pad(p * bstride + x)] = C{va, vb};
     }
     __syncthreads();
-    fft_lds<T, E, false>(buf, a.dx, G, bstride, 1, tw, tid, kThreads);
+    fft_lds<T, E, false>(buf, 0, a.dx, G, 0, bstride, 1, tw, tid, kThreads);
     // Z = A + i B with A, B the spectra of the two real rows:
     // A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / 2i.
     const int nti = 2 * G;
     for (int w = tid; w < nxh * nti; w += kThreads) {
       const int tl = w % nti, k = w / nti;
       const int p = tl >> 1;
-      const C zk = buf[p * bstride + k];
-      const C zn = buf[p * bstride + (k == 0 ? 0 : nx - k)];
+      const C zk = buf[lds_pad(p * bstride + k)];
+      const C zn = buf[lds_pad(p * bstride + (k == 0 ? 0 : nx - k))];
       C o;
       if ((tl & 1) == 0) {
         o = C{(zk.x + zn.x) * half_norm, (zk.y - zn.y) * half_norm};
@@ -193,14 +193,15 @@ __global__ void __launch_bounds__(kThreads) k_cols(ColArgs a) {
   const uint64_t tb = blockIdx.x / nxh;
   const C* __restrict__ src = static_cast<const C*>(a.R) + (tb * nxh + kx) * static_cast<size_t>(H) * Tb;
   const int nin = H * Tb, nall = ny * Tb;
-  for (int w = tid; w < nall; w += kThreads) buf[w] = w < nin ? src[w] : C{0, 0};
+  for (int w = tid; w < nall; w += kThreads) buf[lds_pad(w)] = w < nin ? src[w] : C{0, 0};
   __syncthreads();
   const C* __restrict__ tw = static_cast<const C*>(a.twy);
-  for (int g0 = 0; g0 < Tb; g0 += a.G) fft_lds<T, E, true>(buf + g0, a.dy, a.G, 1, Tb, tw, tid, kThreads);
+  const int logG = __builtin_ctz(a.G);
+  for (int g0 = 0; g0 < Tb; g0 += a.G) fft_lds<T, E, true>(buf, g0, a.dy, a.G, logG, 1, Tb, tw, tid, kThreads);
   C* __restrict__ S = static_cast<C*>(a.S);
   for (int w = tid; w < nall; w += kThreads) {
     const int ti = w % Tb, ky = w / Tb;
-    S[(static_cast<size_t>(ky) * nxh + kx) * a.Lp + tb * Tb + ti] = buf[w];
+    S[(static_cast<size_t>(ky) * nxh + kx) * a.Lp + tb * Tb + ti] = buf[lds_pad(w)];
   }
 }
 
@@ -230,7 +231,7 @@ __global__ void __launch_bounds__(kThreads) k_time(TimeArgs a, int q_per_xcd, in
   if (q >= nq) return;
   const int L = a.L, n = a.dt.n;
   const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(q) * a.Lp;
-  for (int t = tid; t < n; t += kThreads) buf[t] = t < L ? src[t] : C{0, 0};
+  for (int t = tid; t < n; t += kThreads) buf[lds_pad(t)] = t < L ? src[t] : C{0, 0};
   __syncthreads();
 
   const int seg = (L + kThreads - 1) / kThreads;
@@ -238,8 +239,9 @@ __global__ void __launch_bounds__(kThreads) k_time(TimeArgs a, int q_per_xcd, in
   double mr = 0.0, mi = 0.0;
   if constexpr (kSub) {
     for (int t = t0; t < t1; ++t) {
-      mr += static_cast<double>(buf[t].x);
-      mi += static_cast<double>(buf[t].y);
+      const C f = buf[lds_pad(t)];
+      mr += static_cast<double>(f.x);
+      mi += static_cast<double>(f.y);
     }
     mr = block_sum(mr) / L;
     mi = block_sum(mi) / L;
@@ -251,11 +253,11 @@ __global__ void __launch_bounds__(kThreads) k_time(TimeArgs a, int q_per_xcd, in
   for (int i = 0; i < SEG; ++i) {
     const int t = t0 + i;
     if (t < t1) {
-      C f = buf[t];
+      C f = buf[lds_pad(t)];
       if constexpr (kSub) {
         f.x -= m_r;
         f.y -= m_i;
-        buf[t] = f;
+        buf[lds_pad(t)] = f;
       }
       run += static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
       sr += static_cast<Acc>(f.x);
@@ -273,23 +275,24 @@ __global__ void __launch_bounds__(kThreads) k_time(TimeArgs a, int q_per_xcd, in
   __shared__ C edge[2 * kEdge];
   if constexpr (kSub) {
     for (int u = tid; u < kEdge; u += kThreads) {
-      edge[u] = u < L ? buf[u] : C{0, 0};
+      edge[u] = u < L ? buf[lds_pad(u)] : C{0, 0};
       const int s = L - kEdge + u;
-      edge[kEdge + u] = s >= 0 ? buf[s] : C{0, 0};
+      edge[kEdge + u] = s >= 0 ? buf[lds_pad(s)] : C{0, 0};
     }
   }
 
   const C* __restrict__ tw = static_cast<const C*>(a.twt);
-  fft_lds<T, E, false>(buf, a.dt, 1, n, 1, tw, tid, kThreads);
+  fft_lds<T, E, false>(buf, 0, a.dt, 1, 0, n, 1, tw, tid, kThreads);
   for (int t = tid; t < n; t += kThreads) {
-    const C g = buf[t];
-    buf[t] = C{g.x * g.x + g.y * g.y, 0};
+    const C g = buf[lds_pad(t)];
+    buf[lds_pad(t)] = C{g.x * g.x + g.y * g.y, 0};
   }
   __syncthreads();
-  fft_lds<T, E, false>(buf, a.dt, 1, n, 1, tw, tid, kThreads);
+  fft_lds<T, E, false>(buf, 0, a.dt, 1, 0, n, 1, tw, tid, kThreads);
   // Re buf[k] = n * C(k), C(k) = Re sum_t conj F(t) F(t + k).  Prefix sums
-  // psum(k) = sum_{t<k} |F(t)|^2, k = 1 .. L-1, go to the free tail buf[L..n).
-  Acc* tail = reinterpret_cast<Acc*>(buf + L);
+  // psum(k) = sum_{t<k} |F(t)|^2, k = 1 .. L-1, go to the free tail of the
+  // buffer (elements L .. n-1 are no longer needed).
+  Acc* tail = reinterpret_cast<Acc*>(buf + lds_pad(L));
 #pragma unroll
   for (int i = 0; i < SEG; ++i) {
     const int t = t0 + i;
@@ -319,7 +322,7 @@ __global__ void __launch_bounds__(kThreads) k_time(TimeArgs a, int q_per_xcd, in
     } else {
       const Acc head = tail[L - d - 1];   // psum(L - d)
       const Acc tl = ptot - tail[d - 1];  // psum(L) - psum(d)
-      const Acc c = static_cast<Acc>(buf[d].x) * inv_n;
+      const Acc c = static_cast<Acc>(buf[lds_pad(d)].x) * inv_n;
       D = (head + tl - 2 * c) / static_cast<Acc>(L - d);
     }
     out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(D);
@@ -419,10 +422,10 @@ __global__ void __launch_bounds__(kThreads) k_fft_test(cplx_t<T>* data, FftDesc 
   extern __shared__ __align__(16) unsigned char smem[];
   C* buf = reinterpret_cast<C*>(smem);
   C* p = data + static_cast<size_t>(blockIdx.x) * d.n;
-  for (int t = threadIdx.x; t < d.n; t += kThreads) buf[t] = p[t];
+  for (int t = threadIdx.x; t < d.n; t += kThreads) buf[lds_pad(t)] = p[t];
   __syncthreads();
-  fft_lds<T, E, false>(buf, d, 1, d.n, 1, tw, threadIdx.x, kThreads);
-  for (int t = threadIdx.x; t < d.n; t += kThreads) p[t] = buf[t];
+  fft_lds<T, E, false>(buf, 0, d, 1, 0, d.n, 1, tw, threadIdx.x, kThreads);
+  for (int t = threadIdx.x; t < d.n; t += kThreads) p[t] = buf[lds_pad(t)];
 }
 
 // ---------------------------------------------------------------------------
@@ -479,20 +482,20 @@ hipError_t launch_rows(const RowArgs& a, int precision, hipStream_t s) {
   const unsigned grid = static_cast<unsigned>(nblocks * a.H);
   const int e = rc_bucket(static_cast<long long>(a.G) * a.nx);
   if (precision == PREC_FP64) {
-    return rows_t<double>(a, grid, sizeof(double2) * a.G * (a.nx + 1), e, s);
+    return rows_t<double>(a, grid, sizeof(double2) * lds_padded_size(a.G * (a.nx + 1)), e, s);
   }
-  return rows_t<float>(a, grid, sizeof(float2) * a.G * (a.nx + 1), e, s);
+  return rows_t<float>(a, grid, sizeof(float2) * lds_padded_size(a.G * (a.nx + 1)), e, s);
 }
 
 hipError_t launch_cols(const ColArgs& a, int precision, hipStream_t s) {
   const unsigned grid = static_cast<unsigned>(a.nblocks * a.nxh);
   const int e = rc_bucket(static_cast<long long>(a.G) * a.ny);
   if (precision == PREC_FP64) {
-    const size_t lds = sizeof(double2) * a.ny * a.Tb;
+    const size_t lds = sizeof(double2) * lds_padded_size(a.ny * a.Tb);
     if (e == 16) return launch(k_cols<double, 16>, grid, lds, s, a);
     if (e == 32) return launch(k_cols<double, 32>, grid, lds, s, a);
   } else {
-    const size_t lds = sizeof(float2) * a.ny * a.Tb;
+    const size_t lds = sizeof(float2) * lds_padded_size(a.ny * a.Tb);
     if (e == 16) return launch(k_cols<float, 16>, grid, lds, s, a);
     if (e == 32) return launch(k_cols<float, 32>, grid, lds, s, a);
   }
@@ -505,7 +508,7 @@ hipError_t time_e(const TimeArgs& a, int e, hipStream_t s) {
   const int nq = a.ny * a.nxh;
   const int qpx = (nq + 7) / 8;
   const unsigned grid = static_cast<unsigned>(qpx) * 8u;
-  const size_t lds = sizeof(cplx_t<T>) * a.dt.n;
+  const size_t lds = sizeof(cplx_t<T>) * lds_padded_size(a.dt.n);
   switch (e) {
     case 4: return launch(k_time<T, OutT, 4, PREC>, grid, lds, s, a, qpx, nq);
     case 8: return launch(k_time<T, OutT, 8, PREC>, grid, lds, s, a, qpx, nq);
@@ -558,7 +561,7 @@ hipError_t launch_fft_test(void* data, int n, int batch, const FftDesc& d, const
   const int e = fft_bucket(n);
   if (precision == PREC_FP64) {
     using C = double2;
-    const size_t lds = sizeof(C) * n;
+    const size_t lds = sizeof(C) * lds_padded_size(n);
     C* p = static_cast<C*>(data);
     const C* t = static_cast<const C*>(tw);
     switch (e) {
@@ -570,7 +573,7 @@ hipError_t launch_fft_test(void* data, int n, int batch, const FftDesc& d, const
     }
   }
   using C = float2;
-  const size_t lds = sizeof(C) * n;
+  const size_t lds = sizeof(C) * lds_padded_size(n);
   C* p = static_cast<C*>(data);
   const C* t = static_cast<const C*>(tw);
   switch (e) {

@@ -1,0 +1,534 @@
+// Stage (c) + (e) with power-of-two time transforms.
+//
+// D(q, dt) does not depend on the zero-padded transform length N as long as
+// N >= 2L - 1, so the time stage always runs at N = 2^ceil(log2(2L - 1))
+// (N >= 512) with plans fixed at compile time (p2_plan in fddm_common.hpp):
+// three passes per transform with radix 4..32 butterflies, all strides and
+// LDS offsets known to the compiler, and padded LDS addresses (one slot per
+// 32 elements) that stay linear in the butterfly index.
+//
+// Per q, one group of T threads:
+//   1. loads its first-pass inputs F(t), t = u + b T + r N/R0, straight from
+//      HBM (t >= N/2 >= L is the zero padding and is never read);
+//   2. compensated paths: subtracts the temporal mean, records the first and
+//      last kEdge samples, forms prefix sums of |F(t)|^2 in double (chunked
+//      wave scans);
+//   3. FFT1 (R0, R1, R2); the last pass stays in registers, |G|^2 is formed
+//      there and FFT2's first pass (radix R2, same distribution) runs on it;
+//   4. FFT2 (R2, R1, R0); the last pass leaves Re C(t) (times N) in the same
+//      registers as the loaded F(t);
+//   5. writes C and the prefix sums to LDS, then evaluates D for every
+//      requested lag (direct sums when L - dt <= kEdge), PS and VAR, and
+//      stores them fftshifted along y.
+#include <type_traits>
+#include <utility>
+
+#include "fddm_fft.hpp"
+#include "fddm_kernels.hpp"
+
+namespace fddm {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kEdge = 32;
+
+// ---------------------------------------------------------------------------
+// Compile-time cos / sin of 2 pi a / b: exact octant reduction of the
+// rational angle, then Taylor series on [0, pi/4].
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double taylor_sin(double x) {
+  double x2 = x * x, term = x, sum = x;
+  for (int i = 1; i < 14; ++i) {
+    term *= -x2 / ((2.0 * i) * (2.0 * i + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double taylor_cos(double x) {
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int i = 1; i < 14; ++i) {
+    term *= -x2 / ((2.0 * i - 1.0) * (2.0 * i));
+    sum += term;
+  }
+  return sum;
+}
+struct CosSin {
+  double c, s;
+};
+constexpr CosSin cossin2pi(long long a, long long b) {
+  a %= b;
+  if (a < 0) a += b;
+  const long long o = (8 * a) / b;
+  const long long rem = 8 * a - o * b;
+  const double th = kPi / 4 * static_cast<double>(rem) / static_cast<double>(b);
+  const double ph = kPi / 4 * static_cast<double>(b - rem) / static_cast<double>(b);
+  const double ct = taylor_cos(th), st = taylor_sin(th), cp = taylor_cos(ph), sp = taylor_sin(ph);
+  switch (o) {
+    case 0: return {ct, st};
+    case 1: return {sp, cp};
+    case 2: return {-st, ct};
+    case 3: return {-cp, sp};
+    case 4: return {-ct, -st};
+    case 5: return {-sp, -cp};
+    case 6: return {st, -ct};
+    default: return {cp, -sp};
+  }
+}
+
+// a * exp(-2 pi i E / N), constant folded.
+template <typename T, int E, int N>
+__device__ __forceinline__ cplx_t<T> mul_w(cplx_t<T> a) {
+  using C = cplx_t<T>;
+  constexpr int e = ((E % N) + N) % N;
+  constexpr T h = static_cast<T>(0.70710678118654752440084436210485);
+  if constexpr (e == 0) {
+    return a;
+  } else if constexpr (4 * e == N) {
+    return C{a.y, -a.x};
+  } else if constexpr (2 * e == N) {
+    return C{-a.x, -a.y};
+  } else if constexpr (4 * e == 3 * N) {
+    return C{-a.y, a.x};
+  } else if constexpr (8 * e == N) {
+    return C{(a.x + a.y) * h, (a.y - a.x) * h};
+  } else if constexpr (8 * e == 3 * N) {
+    return C{(a.y - a.x) * h, -(a.x + a.y) * h};
+  } else if constexpr (8 * e == 5 * N) {
+    return C{-(a.x + a.y) * h, (a.x - a.y) * h};
+  } else if constexpr (8 * e == 7 * N) {
+    return C{(a.x - a.y) * h, (a.x + a.y) * h};
+  } else {
+    constexpr CosSin w = cossin2pi(e, N);
+    constexpr T c = static_cast<T>(w.c), s = static_cast<T>(w.s);
+    return C{a.x * c + a.y * s, a.y * c - a.x * s};
+  }
+}
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    sfor<I + 1, N>(f);
+  }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void dft(cplx_t<T> (&v)[R]);
+
+// Length R1 R2 DFT in place, natural order in and out (Cooley-Tukey):
+// X[k1 + R1 k2] = sum_n2 W_R2^(n2 k2) W_N^(n2 k1) sum_n1 x[R2 n1 + n2] W_R1^(n1 k1).
+template <typename T, int R1, int R2>
+__device__ __forceinline__ void dft_composite(cplx_t<T> (&v)[R1 * R2]) {
+  using C = cplx_t<T>;
+  constexpr int N = R1 * R2;
+  C a[R2][R1];
+  sfor<0, R2>([&](auto n2c) {
+    constexpr int n2 = decltype(n2c)::value;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) a[n2][n1] = v[R2 * n1 + n2];
+    dft<T, R1>(a[n2]);
+    sfor<1, R1>([&](auto k1c) {
+      constexpr int k1 = decltype(k1c)::value;
+      a[n2][k1] = mul_w<T, n2 * k1, N>(a[n2][k1]);
+    });
+  });
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    C b[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) b[n2] = a[n2][k1];
+    dft<T, R2>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+  }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void dft(cplx_t<T> (&v)[R]) {
+  if constexpr (R <= 8) {
+    butterfly<T, R>(v);
+  } else if constexpr (R == 16) {
+    dft_composite<T, 4, 4>(v);
+  } else {
+    static_assert(R == 32, "radix");
+    dft_composite<T, 8, 4>(v);
+  }
+}
+
+__device__ __forceinline__ int padi(int e) { return e + (e >> 5); }
+constexpr int cpad(int e) { return e + (e >> 5); }
+
+// One Stockham pass with compile-time size: radix R after NS points, T
+// threads per transform, MB butterflies per thread.  Reads x[j + r M],
+// writes y[(j - k) R + k + r NS] (k = j mod NS); with M and NS powers of two
+// the padded addresses are pad(j) + cpad(r M) and pad((j - k) R + k) +
+// cpad(r NS), i.e. one base per butterfly plus immediate offsets.
+template <typename T, int N, int R, int NS, int TPT>
+struct Pass {
+  using C = cplx_t<T>;
+  static constexpr int M = N / R;
+  static constexpr int MB = M / TPT;
+  static_assert(MB >= 1 && M % TPT == 0, "whole butterflies per thread");
+  static_assert(M % 32 == 0, "linear padded reads");
+
+  static __device__ __forceinline__ void compute(const C* buf, C (&v)[MB][R], const C* __restrict__ tw, int u) {
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      const int j = u + b * TPT;
+      const C* p = buf + padi(j);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = p[cpad(r * M)];
+      if constexpr (NS > 1) {
+        const int k = j & (NS - 1);
+        const C* t = tw + k * R;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], t[r]);
+      }
+      dft<T, R>(v[b]);
+    }
+  }
+  static __device__ __forceinline__ void compute_regs(C (&v)[MB][R]) {
+#pragma unroll
+    for (int b = 0; b < MB; ++b) dft<T, R>(v[b]);
+  }
+  static __device__ __forceinline__ void store(C* buf, const C (&v)[MB][R], int u) {
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      const int j = u + b * TPT;
+      const int k = j & (NS - 1);
+      C* p = buf + padi((j - k) * R + k);
+#pragma unroll
+      for (int r = 0; r < R; ++r) p[cpad(r * NS)] = v[b][r];
+    }
+  }
+};
+
+template <typename A>
+__device__ __forceinline__ A wave_sum(A v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename A>
+__device__ __forceinline__ A wave_incl_scan(A v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const A w = __shfl_up(v, o, 64);
+    if (lane >= o) v += w;
+  }
+  return v;
+}
+
+// Sum over the WPT waves of this thread's group; red holds kWaves doubles.
+template <int WPT>
+__device__ __forceinline__ double group_sum(double v, double* red) {
+  const int wave = threadIdx.x >> 6;
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[wave] = v;
+  __syncthreads();
+  const int first = (wave / WPT) * WPT;
+  double s = 0;
+#pragma unroll
+  for (int w = 0; w < WPT; ++w) s += red[first + w];
+  __syncthreads();
+  return s;
+}
+
+// Twiddle table offsets (complex entries): FFT1 passes 1, 2 then FFT2 passes 1, 2;
+// the pass with radix R after NS points holds exp(-2 pi i r k / (NS R)) at k R + r.
+template <int LOGN>
+struct P2Tw {
+  static constexpr P2Plan P = p2_plan(LOGN);
+  static constexpr int N = 1 << LOGN;
+  static constexpr int a1 = 0;
+  static constexpr int a2 = P.r0 * P.r1;
+  static constexpr int b1 = a2 + N;
+  static constexpr int b2 = b1 + P.r2 * P.r1;
+  static constexpr int total = b2 + N;
+};
+
+template <int LOGN, typename T, bool SUB>
+struct P2Lds {
+  static constexpr P2Plan P = p2_plan(LOGN);
+  static constexpr int N = 1 << LOGN, TPT = P.threads, G = kThreads / TPT, WPT = TPT / 64;
+  static constexpr int MB0 = N / P.r0 / TPT;
+  static constexpr int NCH = (P.r0 / 2) * MB0;
+  using Acc = std::conditional_t<SUB, double, float>;
+  static constexpr size_t csz = sizeof(cplx_t<T>);
+  static constexpr size_t buf = static_cast<size_t>(G) * lds_padded_size(N) * csz;
+  static constexpr size_t edge = static_cast<size_t>(G) * 2 * kEdge * csz;
+  static constexpr size_t red = kWaves * sizeof(double);
+  static constexpr size_t chunk = static_cast<size_t>(G) * NCH * WPT * sizeof(Acc);
+  static constexpr size_t bytes = buf + edge + red + chunk;
+};
+
+template <typename T, typename OutT, int LOGN, bool SUB>
+__global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd, int nq) {
+  using C = cplx_t<T>;
+  using Lay = P2Lds<LOGN, T, SUB>;
+  using Acc = typename Lay::Acc;
+  using Tw = P2Tw<LOGN>;
+  constexpr P2Plan P = p2_plan(LOGN);
+  constexpr int N = 1 << LOGN, TPT = P.threads, G = Lay::G, WPT = Lay::WPT;
+  constexpr int R0 = P.r0, R1 = P.r1, R2 = P.r2;
+  using A0 = Pass<T, N, R0, 1, TPT>;
+  using A1 = Pass<T, N, R1, R0, TPT>;
+  using A2 = Pass<T, N, R2, R0 * R1, TPT>;
+  using B0 = Pass<T, N, R2, 1, TPT>;
+  using B1 = Pass<T, N, R1, R2, TPT>;
+  using B2 = Pass<T, N, R0, R2 * R1, TPT>;
+  static_assert(A2::MB == B0::MB && B2::MB == A0::MB, "register hand-offs");
+  constexpr int MB0 = A0::MB, M0 = A0::M, NCH = Lay::NCH, RH = R0 / 2;
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x;
+  const int g = tid / TPT, u = tid - g * TPT, lane = tid & 63, wg = u >> 6;
+  C* const buf = reinterpret_cast<C*>(smem) + g * lds_padded_size(N);
+  C* const edge = reinterpret_cast<C*>(smem + Lay::buf) + g * 2 * kEdge;
+  double* const red = reinterpret_cast<double*>(smem + Lay::buf + Lay::edge);
+  Acc* const chunk = reinterpret_cast<Acc*>(smem + Lay::buf + Lay::edge + Lay::red) + g * NCH * WPT;
+
+  // Workgroups are dealt round-robin to the 8 XCDs: each XCD gets a
+  // contiguous range of q so neighbouring output columns share its L2.
+  const int q = ((blockIdx.x & 7) * wg_per_xcd + (blockIdx.x >> 3)) * G + g;
+  const bool active = q < nq;
+  const int L = a.L;
+  const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(active ? q : 0) * a.Lp;
+  const C* __restrict__ tw = static_cast<const C*>(a.twt);
+
+  // 1. first-pass inputs straight from HBM
+  C v0[MB0][R0];
+#pragma unroll
+  for (int b = 0; b < MB0; ++b)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int t = u + b * TPT + r * M0;
+      v0[b][r] = (r < RH && active && t < L) ? src[t] : C{0, 0};
+    }
+
+  // 2. mean, subtraction, edges, prefix sums of |F|^2
+  double sr = 0.0, si = 0.0;
+#pragma unroll
+  for (int b = 0; b < MB0; ++b)
+#pragma unroll
+    for (int r = 0; r < RH; ++r) {
+      sr += static_cast<double>(v0[b][r].x);
+      si += static_cast<double>(v0[b][r].y);
+    }
+  sr = group_sum<WPT>(sr, red);
+  si = group_sum<WPT>(si, red);
+  const double mr = sr / L, mi = si / L;
+  const T m_r = SUB ? static_cast<T>(mr) : T(0), m_i = SUB ? static_cast<T>(mi) : T(0);
+  Acc incl[NCH];
+#pragma unroll
+  for (int b = 0; b < MB0; ++b)
+#pragma unroll
+    for (int r = 0; r < RH; ++r) {
+      const int t = u + b * TPT + r * M0;
+      Acc val = 0;
+      if (t < L) {
+        C f = v0[b][r];
+        if constexpr (SUB) {
+          f.x -= m_r;
+          f.y -= m_i;
+          v0[b][r] = f;
+          if (t < kEdge) edge[t] = f;
+          if (t >= L - kEdge) edge[t - L + 2 * kEdge] = f;
+        }
+        val = static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
+      }
+      incl[r * MB0 + b] = wave_incl_scan(val);
+    }
+  // chunk c covers t = c T .. c T + T - 1; combine the wave scans
+#pragma unroll
+  for (int c = 0; c < NCH; ++c)
+    if (lane == 63) chunk[c * WPT + wg] = incl[c];
+  __syncthreads();
+  Acc ptot = 0;
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    Acc tot = 0, pre = 0;
+#pragma unroll
+    for (int w = 0; w < WPT; ++w) {
+      const Acc x = chunk[c * WPT + w];
+      tot += x;
+      if (w < wg) pre += x;
+    }
+    incl[c] += ptot + pre;  // psum(t + 1)
+    ptot += tot;
+  }
+  // sum of the (subtracted) series, for VAR and PS
+  const double fsr = sr - static_cast<double>(L) * static_cast<double>(m_r);
+  const double fsi = si - static_cast<double>(L) * static_cast<double>(m_i);
+
+  // 3. FFT1; its last pass stays in registers
+  A0::compute_regs(v0);
+  A0::store(buf, v0, u);
+  __syncthreads();
+  C v1[A1::MB][R1];
+  A1::compute(buf, v1, tw + Tw::a1, u);
+  __syncthreads();
+  A1::store(buf, v1, u);
+  __syncthreads();
+  C v2[A2::MB][R2];
+  A2::compute(buf, v2, tw + Tw::a2, u);
+#pragma unroll
+  for (int b = 0; b < A2::MB; ++b)
+#pragma unroll
+    for (int r = 0; r < R2; ++r) v2[b][r] = C{v2[b][r].x * v2[b][r].x + v2[b][r].y * v2[b][r].y, 0};
+  // 4. FFT2; first pass on the registers, last pass back into registers
+  B0::compute_regs(v2);
+  __syncthreads();
+  B0::store(buf, v2, u);
+  __syncthreads();
+  B1::compute(buf, v1, tw + Tw::b1, u);
+  __syncthreads();
+  B1::store(buf, v1, u);
+  __syncthreads();
+  B2::compute(buf, v0, tw + Tw::b2, u);
+  __syncthreads();
+
+  // 5. C(t) N and psum(t + 1) into LDS
+  Acc* const tail = reinterpret_cast<Acc*>(buf + padi(L));
+#pragma unroll
+  for (int b = 0; b < MB0; ++b)
+#pragma unroll
+    for (int r = 0; r < RH; ++r) {
+      const int t = u + b * TPT + r * M0;
+      if (t < L) buf[padi(t)].x = v0[b][r].x;
+      if (t < L - 1) tail[t] = incl[r * MB0 + b];
+    }
+  __syncthreads();
+  if (!active) return;
+
+  OutT* __restrict__ out = static_cast<OutT*>(a.out);
+  const size_t nqs = static_cast<size_t>(a.ny) * a.nxh;
+  const int ky = q / a.nxh, kx = q - ky * a.nxh;
+  const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
+  const Acc inv_n = static_cast<Acc>(1) / static_cast<Acc>(N);
+  for (int i = u; i < a.nl; i += TPT) {
+    const int d = static_cast<int>(a.lags[i]);
+    Acc D;
+    if (SUB && L - d <= kEdge) {
+      Acc acc = 0;
+      for (int t = 0; t < L - d; ++t) {
+        const int s = t + d;
+        const C x = s < kEdge ? edge[s] : edge[s - L + 2 * kEdge];
+        const C y = edge[t];
+        const Acc dr = static_cast<Acc>(x.x) - static_cast<Acc>(y.x);
+        const Acc di = static_cast<Acc>(x.y) - static_cast<Acc>(y.y);
+        acc += dr * dr + di * di;
+      }
+      D = acc / static_cast<Acc>(L - d);
+    } else {
+      const Acc head = tail[L - d - 1];   // psum(L - d)
+      const Acc tl = ptot - tail[d - 1];  // psum(L) - psum(d)
+      const Acc c = static_cast<Acc>(buf[padi(d)].x) * inv_n;
+      D = (head + tl - 2 * c) / static_cast<Acc>(L - d);
+    }
+    out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(D);
+  }
+  if (u == 0) {
+    const Acc mr2 = static_cast<Acc>(fsr / L), mi2 = static_cast<Acc>(fsi / L);
+    Acc ps, var;
+    if constexpr (SUB) {
+      var = ptot / L - (mr2 * mr2 + mi2 * mi2);
+      Acc fr = static_cast<Acc>(m_r) + mr2, fi = static_cast<Acc>(m_i) + mi2;
+      if (a.S0) {
+        const C z = static_cast<const C*>(a.S0)[static_cast<size_t>(q) * a.s0_stride];
+        fr += static_cast<Acc>(z.x);
+        fi += static_cast<Acc>(z.y);
+      }
+      ps = var + fr * fr + fi * fi;
+    } else {
+      ps = ptot / L;
+      var = ps - (mr2 * mr2 + mi2 * mi2);
+    }
+    out[static_cast<size_t>(a.nl) * nqs + oq] = static_cast<OutT>(ps);
+    out[static_cast<size_t>(a.nl + 1) * nqs + oq] = static_cast<OutT>(var);
+  }
+}
+
+template <typename T, typename OutT, int LOGN, bool SUB>
+hipError_t launch_one(const TimeArgs& a, hipStream_t s) {
+  using Lay = P2Lds<LOGN, T, SUB>;
+  const int nq = a.ny * a.nxh;
+  const int wgs = (nq + Lay::G - 1) / Lay::G;
+  const int wpx = (wgs + 7) / 8;
+  const size_t lds = Lay::bytes;
+  auto kernel = k_time_p2<T, OutT, LOGN, SUB>;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kernel, dim3(8u * wpx), dim3(kThreads), lds, s, a, wpx, nq);
+  return hipGetLastError();
+}
+
+template <typename T, typename OutT, bool SUB>
+hipError_t launch_logn(const TimeArgs& a, int logn, hipStream_t s) {
+  switch (logn) {
+    case 9: return launch_one<T, OutT, 9, SUB>(a, s);
+    case 10: return launch_one<T, OutT, 10, SUB>(a, s);
+    case 11: return launch_one<T, OutT, 11, SUB>(a, s);
+    case 12: return launch_one<T, OutT, 12, SUB>(a, s);
+    case 13: return launch_one<T, OutT, 13, SUB>(a, s);
+    case 14:
+      if constexpr (sizeof(T) == 4) return launch_one<T, OutT, 14, SUB>(a, s);
+      return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+size_t p2_lds_bytes(int logn, int precision) {
+  auto pick = [&](auto tag, auto sub) -> size_t {
+    using T = decltype(tag);
+    constexpr bool S = decltype(sub)::value;
+    switch (logn) {
+      case 9: return P2Lds<9, T, S>::bytes;
+      case 10: return P2Lds<10, T, S>::bytes;
+      case 11: return P2Lds<11, T, S>::bytes;
+      case 12: return P2Lds<12, T, S>::bytes;
+      case 13: return P2Lds<13, T, S>::bytes;
+      case 14: return P2Lds<14, T, S>::bytes;
+      default: return 0;
+    }
+  };
+  if (precision == PREC_FP64) return pick(double{}, std::true_type{});
+  if (precision == PREC_FP32) return pick(float{}, std::false_type{});
+  return pick(float{}, std::true_type{});
+}
+
+int p2_twiddle_count(int logn) {
+  switch (logn) {
+    case 9: return P2Tw<9>::total;
+    case 10: return P2Tw<10>::total;
+    case 11: return P2Tw<11>::total;
+    case 12: return P2Tw<12>::total;
+    case 13: return P2Tw<13>::total;
+    case 14: return P2Tw<14>::total;
+    default: return 0;
+  }
+}
+
+hipError_t launch_time_p2(const TimeArgs& a, int logn, int precision, int out_dtype, hipStream_t s) {
+  const bool f32 = out_dtype == OUT_F32;
+  switch (precision) {
+    case PREC_FP64:
+      return f32 ? launch_logn<double, float, true>(a, logn, s) : launch_logn<double, double, true>(a, logn, s);
+    case PREC_FP32:
+      return f32 ? launch_logn<float, float, false>(a, logn, s) : launch_logn<float, double, false>(a, logn, s);
+    case PREC_FP32C:
+      return f32 ? launch_logn<float, float, true>(a, logn, s) : launch_logn<float, double, true>(a, logn, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace fddm

@@ -123,6 +123,7 @@ int fddm_malloc(void **ptr, uint64_t bytes, char *err, size_t errlen);
 int fddm_free(void *ptr);
 int fddm_memcpy_htod(void *dst, const void *src, uint64_t bytes, char *err, size_t errlen);
 int fddm_memcpy_dtoh(void *dst, const void *src, uint64_t bytes, char *err, size_t errlen);
+int fddm_memcpy_dtod(void *dst, const void *src, uint64_t bytes, char *err, size_t errlen);
 int fddm_stream_create(void **stream, char *err, size_t errlen);
 int fddm_stream_destroy(void *stream);
 int fddm_stream_synchronize(void *stream, char *err, size_t errlen);
