@@ -174,6 +174,9 @@ struct Pass {
   static_assert(M % 32 == 0, "linear padded reads");
 
   static __device__ __forceinline__ void compute(const C* buf, C (&v)[MB][R], const C* __restrict__ tw, int u) {
+    // opaque thread index: the address arithmetic of each pass stays in that
+    // pass instead of being hoisted to the kernel entry and kept live
+    asm volatile("" : "+v"(u));
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
       const int j = u + b * TPT;
@@ -194,6 +197,7 @@ struct Pass {
     for (int b = 0; b < MB; ++b) dft<T, R>(v[b]);
   }
   static __device__ __forceinline__ void store(C* buf, const C (&v)[MB][R], int u) {
+    asm volatile("" : "+v"(u));
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
       const int j = u + b * TPT;
@@ -262,7 +266,7 @@ struct P2Lds {
   static constexpr size_t buf = static_cast<size_t>(G) * lds_padded_size(N) * csz;
   static constexpr size_t edge = static_cast<size_t>(G) * 2 * kEdge * csz;
   static constexpr size_t red = kWaves * sizeof(double);
-  static constexpr size_t chunk = static_cast<size_t>(G) * NCH * WPT * sizeof(Acc);
+  static constexpr size_t chunk = static_cast<size_t>(G) * WPT * sizeof(Acc);
   static constexpr size_t bytes = buf + edge + red + chunk;
 };
 
@@ -282,7 +286,7 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
   using B1 = Pass<T, N, R1, R2, TPT>;
   using B2 = Pass<T, N, R0, R2 * R1, TPT>;
   static_assert(A2::MB == B0::MB && B2::MB == A0::MB, "register hand-offs");
-  constexpr int MB0 = A0::MB, M0 = A0::M, NCH = Lay::NCH, RH = R0 / 2;
+  constexpr int MB0 = A0::MB, M0 = A0::M, RH = R0 / 2;
 
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
@@ -290,7 +294,7 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
   C* const buf = reinterpret_cast<C*>(smem) + g * lds_padded_size(N);
   C* const edge = reinterpret_cast<C*>(smem + Lay::buf) + g * 2 * kEdge;
   double* const red = reinterpret_cast<double*>(smem + Lay::buf + Lay::edge);
-  Acc* const chunk = reinterpret_cast<Acc*>(smem + Lay::buf + Lay::edge + Lay::red) + g * NCH * WPT;
+  Acc* const chunk = reinterpret_cast<Acc*>(smem + Lay::buf + Lay::edge + Lay::red) + g * WPT;
 
   // Workgroups are dealt round-robin to the 8 XCDs: each XCD gets a
   // contiguous range of q so neighbouring output columns share its L2.
@@ -310,7 +314,7 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
       v0[b][r] = (r < RH && active && t < L) ? src[t] : C{0, 0};
     }
 
-  // 2. mean, subtraction, edges, prefix sums of |F|^2
+  // 2. temporal mean (the compensated paths transform F - mean)
   double sr = 0.0, si = 0.0;
 #pragma unroll
   for (int b = 0; b < MB0; ++b)
@@ -323,43 +327,17 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
   si = group_sum<WPT>(si, red);
   const double mr = sr / L, mi = si / L;
   const T m_r = SUB ? static_cast<T>(mr) : T(0), m_i = SUB ? static_cast<T>(mi) : T(0);
-  Acc incl[NCH];
+  if constexpr (SUB) {
 #pragma unroll
-  for (int b = 0; b < MB0; ++b)
+    for (int b = 0; b < MB0; ++b)
 #pragma unroll
-    for (int r = 0; r < RH; ++r) {
-      const int t = u + b * TPT + r * M0;
-      Acc val = 0;
-      if (t < L) {
-        C f = v0[b][r];
-        if constexpr (SUB) {
-          f.x -= m_r;
-          f.y -= m_i;
-          v0[b][r] = f;
-          if (t < kEdge) edge[t] = f;
-          if (t >= L - kEdge) edge[t - L + 2 * kEdge] = f;
+      for (int r = 0; r < RH; ++r) {
+        const int t = u + b * TPT + r * M0;
+        if (t < L) {
+          v0[b][r].x -= m_r;
+          v0[b][r].y -= m_i;
         }
-        val = static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
       }
-      incl[r * MB0 + b] = wave_incl_scan(val);
-    }
-  // chunk c covers t = c T .. c T + T - 1; combine the wave scans
-#pragma unroll
-  for (int c = 0; c < NCH; ++c)
-    if (lane == 63) chunk[c * WPT + wg] = incl[c];
-  __syncthreads();
-  Acc ptot = 0;
-#pragma unroll
-  for (int c = 0; c < NCH; ++c) {
-    Acc tot = 0, pre = 0;
-#pragma unroll
-    for (int w = 0; w < WPT; ++w) {
-      const Acc x = chunk[c * WPT + w];
-      tot += x;
-      if (w < wg) pre += x;
-    }
-    incl[c] += ptot + pre;  // psum(t + 1)
-    ptot += tot;
   }
   // sum of the (subtracted) series, for VAR and PS
   const double fsr = sr - static_cast<double>(L) * static_cast<double>(m_r);
@@ -392,7 +370,10 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
   B2::compute(buf, v0, tw + Tw::b2, u);
   __syncthreads();
 
-  // 5. C(t) N and psum(t + 1) into LDS
+  // 5. C(t) N into LDS; prefix sums psum(t + 1) of |F(t) - mean|^2 from a
+  // second (cache-resident) read of the series, in chunks of T consecutive t
+  // (wave scans combined across the group's waves), into the free tail of the
+  // buffer; first and last kEdge samples into `edge`.
   Acc* const tail = reinterpret_cast<Acc*>(buf + padi(L));
 #pragma unroll
   for (int b = 0; b < MB0; ++b)
@@ -400,9 +381,36 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
     for (int r = 0; r < RH; ++r) {
       const int t = u + b * TPT + r * M0;
       if (t < L) buf[padi(t)].x = v0[b][r].x;
-      if (t < L - 1) tail[t] = incl[r * MB0 + b];
     }
-  __syncthreads();
+  Acc ptot = 0;
+  for (int c0 = 0; c0 < L; c0 += TPT) {
+    const int t = c0 + u;
+    Acc val = 0;
+    if (active && t < L) {
+      C f = src[t];
+      f.x -= m_r;
+      f.y -= m_i;
+      if constexpr (SUB) {
+        if (t < kEdge) edge[t] = f;
+        if (t >= L - kEdge) edge[t - L + 2 * kEdge] = f;
+      }
+      val = static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
+    }
+    Acc inc = wave_incl_scan(val);
+    if (lane == 63) chunk[wg] = inc;
+    __syncthreads();
+    Acc pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < WPT; ++w) {
+      const Acc x = chunk[w];
+      tot += x;
+      if (w < wg) pre += x;
+    }
+    inc += ptot + pre;
+    if (t < L - 1) tail[t] = inc;
+    ptot += tot;
+    __syncthreads();
+  }
   if (!active) return;
 
   OutT* __restrict__ out = static_cast<OutT*>(a.out);
