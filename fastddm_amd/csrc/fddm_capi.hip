@@ -28,6 +28,8 @@ struct fddm_plan {
   uint64_t L = 0, H = 0, W = 0, nx = 0, ny = 0, nt = 0, nxh = 0, nq = 0, nl = 0;
   int Tb = 2, Gx = 1, Gy = 1;
   int logn_t = 0;  // time transform length 2^logn_t (fft mode)
+  int logn_x = 0;  // > 0: power-of-two row kernel
+  int logn_y = 0;  // > 0: power-of-two column kernel
   uint64_t Lp = 0, nblocks = 0;
   bool subtract0 = false;
   FftDesc dx{}, dy{}, dt{};
@@ -169,6 +171,32 @@ std::vector<T> p2_twiddles(int logn) {
   return w;
 }
 
+// Twiddles of a three-pass power-of-two spatial plan: passes 1 and 2,
+// [k][r] = exp(-2 pi i r k / (NS R)).
+template <typename T>
+std::vector<T> sp_twiddles(int r0, int r1, int r2) {
+  std::vector<T> w;
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  auto pass = [&](long long R, long long NS) {
+    for (long long k = 0; k < NS; ++k)
+      for (long long r = 0; r < R; ++r) {
+        const long double ang = -two_pi * static_cast<long double>(r * k) / static_cast<long double>(NS * R);
+        w.push_back(static_cast<T>(cosl(ang)));
+        w.push_back(static_cast<T>(sinl(ang)));
+      }
+  };
+  pass(r1, r0);
+  pass(r2, static_cast<long long>(r0) * r1);
+  return w;
+}
+
+int ilog2_exact(uint64_t n) {
+  if (n == 0 || (n & (n - 1))) return 0;
+  int l = 0;
+  while ((1ull << l) < n) ++l;
+  return l;
+}
+
 int upload(void** dst, const void* src, size_t bytes, char* err, size_t errlen) {
   FDDM_HIP(hipMalloc(dst, bytes ? bytes : 1));
   if (bytes) FDDM_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
@@ -280,9 +308,29 @@ int plan_create(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H, ui
     p->logn_t = logn;
   }
 
+  // Power-of-two spatial kernels where the size and frame block allow them.
+  {
+    const int lx = ilog2_exact(nx), ly = ilog2_exact(ny);
+    if (p2_spatial_ok(lx) && (p->Tb == 8 || p->Tb == 16) &&
+        lds_padded_size((p->Tb / 2) * static_cast<int>(nx)) * csz <= 96 * 1024)
+      p->logn_x = lx;
+    if (p2_spatial_ok(ly) && p->Tb >= 4) p->logn_y = ly;
+  }
   int rc;
-  if ((rc = upload_twiddles(&p->d_twx, p->dx, f64, err, errlen))) return fail(rc);
-  if ((rc = upload_twiddles(&p->d_twy, p->dy, f64, err, errlen))) return fail(rc);
+  auto upload_sp = [&](void** dst, int logn) {
+    int r0, r1, r2;
+    p2_spatial_plan(logn, &r0, &r1, &r2);
+    if (f64) {
+      auto w = sp_twiddles<double>(r0, r1, r2);
+      return upload(dst, w.data(), w.size() * sizeof(double), err, errlen);
+    }
+    auto w = sp_twiddles<float>(r0, r1, r2);
+    return upload(dst, w.data(), w.size() * sizeof(float), err, errlen);
+  };
+  if ((rc = p->logn_x ? upload_sp(&p->d_twx, p->logn_x) : upload_twiddles(&p->d_twx, p->dx, f64, err, errlen)))
+    return fail(rc);
+  if ((rc = p->logn_y ? upload_sp(&p->d_twy, p->logn_y) : upload_twiddles(&p->d_twy, p->dy, f64, err, errlen)))
+    return fail(rc);
   if (mode == MODE_FFT) {
     if (f64) {
       auto w = p2_twiddles<double>(p->logn_t);
@@ -344,6 +392,7 @@ int run_spatial(fddm_plan* p, const void* d_img, void* ws, hipStream_t s, char* 
   ra.Tb = p->Tb;
   ra.G = p->Gx;
   ra.window = p->d_window;
+  ra.logn = p->logn_x;
   ra.norm = norm;
   ra.dx = p->dx;
   ra.twx = p->d_twx;
@@ -355,6 +404,7 @@ int run_spatial(fddm_plan* p, const void* d_img, void* ws, hipStream_t s, char* 
   ca.nxh = static_cast<int>(p->nxh);
   ca.Tb = p->Tb;
   ca.G = p->Gy;
+  ca.logn = p->logn_y;
   ca.dy = p->dy;
   ca.twy = p->d_twy;
   int rc;
@@ -363,21 +413,21 @@ int run_spatial(fddm_plan* p, const void* d_img, void* ws, hipStream_t s, char* 
     // F(0) of the raw first frame, for the power spectrum.
     ra.frames = 1;
     ra.subtract0 = 0;
-    FDDM_HIP(launch_rows(ra, p->precision, s));
+    FDDM_HIP(ra.logn ? launch_rows_p2(ra, p->precision, s) : launch_rows(ra, p->precision, s));
     ca.nblocks = 1;
     ca.Lp = static_cast<uint64_t>(p->Tb);
     ca.S = S0;
-    FDDM_HIP(launch_cols(ca, p->precision, s));
+    FDDM_HIP(ca.logn ? launch_cols_p2(ca, p->precision, s) : launch_cols(ca, p->precision, s));
   }
   if ((rc = record(p, 1, s, err, errlen))) return rc;
   ra.frames = p->L;
   ra.subtract0 = p->subtract0 ? 1 : 0;
-  FDDM_HIP(launch_rows(ra, p->precision, s));
+  FDDM_HIP(ra.logn ? launch_rows_p2(ra, p->precision, s) : launch_rows(ra, p->precision, s));
   if ((rc = record(p, 2, s, err, errlen))) return rc;
   ca.nblocks = p->nblocks;
   ca.Lp = p->Lp;
   ca.S = S;
-  FDDM_HIP(launch_cols(ca, p->precision, s));
+  FDDM_HIP(ca.logn ? launch_cols_p2(ca, p->precision, s) : launch_cols(ca, p->precision, s));
   if ((rc = record(p, 3, s, err, errlen))) return rc;
   return FDDM_OK;
 }

@@ -16,6 +16,7 @@ struct RowArgs {
   uint64_t L;       // frames present in img
   uint64_t frames;  // frames to transform (<= L)
   int H, W, nx, Tb, G;
+  int logn;  // > 0: power-of-two kernel for nx = 2^logn (twx holds its tables)
   const void* window;
   int subtract0;
   double norm;
@@ -30,6 +31,7 @@ struct ColArgs {
   const void* R;
   uint64_t nblocks;
   int H, ny, nxh, Tb, G;
+  int logn;  // > 0: power-of-two kernel for ny = 2^logn (twy holds its tables)
   uint64_t Lp;
   FftDesc dy;
   const void* twy;
@@ -60,6 +62,11 @@ hipError_t launch_diff(const TimeArgs& a, int precision, int out_dtype, hipStrea
 size_t p2_lds_bytes(int logn, int precision);
 int p2_twiddle_count(int logn);
 hipError_t launch_time_p2(const TimeArgs& a, int logn, int precision, int out_dtype, hipStream_t s);
+// Power-of-two spatial kernels (256 <= n <= 4096): plan radices and launchers.
+bool p2_spatial_ok(int logn);
+int p2_spatial_plan(int logn, int* r0, int* r1, int* r2);
+hipError_t launch_rows_p2(const RowArgs& a, int precision, hipStream_t s);
+hipError_t launch_cols_p2(const ColArgs& a, int precision, hipStream_t s);
 hipError_t launch_fft_test(void* data, int n, int batch, const FftDesc& d, const void* tw,
                            int precision, hipStream_t s);
 

@@ -1,0 +1,268 @@
+// Stage (a)+(b) for power-of-two image sizes (256 .. 4096): the same data
+// flow as k_rows / k_cols (fddm_kernels.hip) with compile-time three-pass
+// plans (p2::sp_plan), constant LDS offsets and constant twiddle strides.
+//
+//   k_rows_p2  Tb frames of one image row: frame pairs (2p, 2p+1) packed as
+//              one complex row, convert (+ frame-0 subtraction) x window,
+//              length-NX FFTs of all Tb/2 pairs at once, split into half spectra,
+//              store R[tb][kx][y][ti] with the frame index fastest.
+//   k_cols_p2  one kx column of Tb frames: length-NY FFTs of the interleaved
+//              slab [y][ti]; the last pass writes the q-major spectra
+//              S[ky * nxh + kx][t] straight from registers.
+#include "fddm_kernels.hpp"
+#include "fddm_p2.hpp"
+
+namespace fddm {
+namespace {
+using namespace p2;
+
+// 512 threads per workgroup: the LDS slabs (up to 64 KiB) bound the number
+// of resident workgroups, so each one brings 8 waves.
+constexpr int kThreads = 512;
+
+template <typename Tin, typename T>
+__device__ __forceinline__ T pixel(const Tin* __restrict__ img, size_t idx, size_t idx0, bool sub) {
+  if constexpr (std::is_integral_v<Tin> && sizeof(Tin) <= 2) {
+    int v = static_cast<int>(img[idx]);
+    if (sub) v -= static_cast<int>(img[idx0]);
+    return static_cast<T>(v);
+  } else if constexpr (std::is_integral_v<Tin> && sizeof(Tin) == 4) {
+    long long v = static_cast<long long>(img[idx]);
+    if (sub) v -= static_cast<long long>(img[idx0]);
+    return static_cast<T>(v);
+  } else if constexpr (std::is_integral_v<Tin>) {
+    double v = static_cast<double>(img[idx]);
+    if (sub) v -= static_cast<double>(img[idx0]);
+    return static_cast<T>(v);
+  } else {
+    Tin v = img[idx];
+    if (sub) v -= img[idx0];
+    return static_cast<T>(v);
+  }
+}
+
+template <typename Tin, typename T, int LOGX, int NP>
+__global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a) {
+  using C = cplx_t<T>;
+  constexpr int NX = 1 << LOGX, NXH = NX / 2 + 1;
+  constexpr SpPlan P = sp_plan(LOGX);
+  constexpr int XPT = (NX + kThreads - 1) / kThreads;  // x positions per thread
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* const buf = reinterpret_cast<C*>(smem);
+  const int tid = threadIdx.x;
+  const int H = a.H, W = a.W, Tb = a.Tb;
+  const int y = blockIdx.x % H;
+  const uint64_t tb = blockIdx.x / H;
+  const uint64_t t0 = tb * Tb;
+  const Tin* __restrict__ img = static_cast<const Tin*>(a.img);
+  const T* __restrict__ win = static_cast<const T*>(a.window);
+  const size_t plane = static_cast<size_t>(H) * W;
+  const bool sub = a.subtract0 != 0;
+  const C* __restrict__ tw = static_cast<const C*>(a.twx);
+  constexpr int npairs = NP;  // == Tb / 2
+
+  // window values of this thread's x positions (the same for every pair)
+  T wv[XPT];
+#pragma unroll
+  for (int i = 0; i < XPT; ++i) {
+    const int x = tid + i * kThreads;
+    wv[i] = (win && x < W) ? win[static_cast<size_t>(y) * W + x] : T(1);
+  }
+  // pairs -> LDS, element (p, x) at logical p NX + x
+  for (int p = 0; p < npairs; ++p) {
+    const uint64_t ta = t0 + 2 * p, tb2 = ta + 1;
+#pragma unroll
+    for (int i = 0; i < XPT; ++i) {
+      const int x = tid + i * kThreads;
+      if (x < NX) {
+        T va = 0, vb = 0;
+        if (x < W) {
+          const size_t i0 = static_cast<size_t>(y) * W + x;
+          if (ta < a.frames) va = pixel<Tin, T>(img, ta * plane + i0, i0, sub) * wv[i];
+          if (tb2 < a.frames) vb = pixel<Tin, T>(img, tb2 * plane + i0, i0, sub) * wv[i];
+        }
+        buf[padi(p * NX + x)] = C{va, vb};
+      }
+    }
+  }
+  __syncthreads();
+  fft3_lds<T, NX, P.r0, P.r1, P.r2, kThreads, NP, false, 1, NX>(buf, tw, tid);
+
+  // A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / 2i; store
+  // R[tb][k][y][ti] with ti fastest
+  const T half_norm = static_cast<T>(0.5 * a.norm);
+  C* __restrict__ R = static_cast<C*>(a.R);
+  for (int w = tid; w < NXH * Tb; w += kThreads) {
+    const int ti = w % Tb, k = w / Tb;
+    const int p = ti >> 1;
+    const C zk = buf[padi(p * NX + k)];
+    const C zn = buf[padi(p * NX + ((NX - k) & (NX - 1)))];
+    const C o = (ti & 1) == 0 ? C{(zk.x + zn.x) * half_norm, (zk.y - zn.y) * half_norm}
+                              : C{(zk.y + zn.y) * half_norm, (zn.x - zk.x) * half_norm};
+    R[((tb * NXH + k) * static_cast<size_t>(H) + y) * Tb + ti] = o;
+  }
+}
+
+template <typename T, int LOGY, int TB>
+__global__ void __launch_bounds__(kThreads) k_cols_p2(ColArgs a) {
+  using C = cplx_t<T>;
+  constexpr int NY = 1 << LOGY;
+  constexpr SpPlan P = sp_plan(LOGY);
+  using A0 = Pass<T, NY, P.r0, 1, kThreads, TB, true, TB, 1>;
+  using A1 = Pass<T, NY, P.r1, P.r0, kThreads, TB, true, TB, 1>;
+  using A2 = Pass<T, NY, P.r2, P.r0 * P.r1, kThreads, TB, true, TB, 1>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* const buf = reinterpret_cast<C*>(smem);
+  const int tid = threadIdx.x;
+  const int nxh = a.nxh, H = a.H;
+  const int kx = blockIdx.x % nxh;
+  const uint64_t tb = blockIdx.x / nxh;
+  const C* __restrict__ src = static_cast<const C*>(a.R) + (tb * nxh + kx) * static_cast<size_t>(H) * TB;
+  const int nin = H * TB;
+  for (int e = tid; e < NY * TB; e += kThreads) buf[padi(e)] = e < nin ? src[e] : C{0, 0};
+  __syncthreads();
+  const C* __restrict__ tw = static_cast<const C*>(a.twy);
+  {
+    C v[A0::MB][P.r0];
+    A0::compute(buf, v, tw, tid);
+    __syncthreads();
+    A0::store(buf, v, tid);
+    __syncthreads();
+  }
+  {
+    C v[A1::MB][P.r1];
+    A1::compute(buf, v, tw, tid);
+    __syncthreads();
+    A1::store(buf, v, tid);
+    __syncthreads();
+  }
+  // last pass: outputs ky = j + r NY/R2 of frame ti go straight to S
+  C v[A2::MB][P.r2];
+  A2::compute(buf, v, tw + P.r0 * P.r1, tid);
+  C* __restrict__ S = static_cast<C*>(a.S);
+  constexpr int M2 = NY / P.r2;
+#pragma unroll
+  for (int b = 0; b < A2::MB; ++b) {
+    const int w = tid + b * kThreads;
+    if (A2::ITEMS % kThreads != 0 && w >= A2::ITEMS) break;
+    const int ti = w % TB, j = w / TB;
+#pragma unroll
+    for (int r = 0; r < P.r2; ++r) {
+      const int ky = j + r * M2;
+      S[(static_cast<size_t>(ky) * nxh + kx) * a.Lp + tb * TB + ti] = v[b][r];
+    }
+  }
+}
+
+template <int LOGX, typename Tin, typename T, int NP>
+hipError_t rows_np(const RowArgs& a, unsigned grid, hipStream_t s) {
+  const size_t lds = sizeof(cplx_t<T>) * lds_padded_size(NP * (1 << LOGX));
+  auto kernel = k_rows_p2<Tin, T, LOGX, NP>;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), lds, s, a);
+  return hipGetLastError();
+}
+
+template <int LOGX, typename Tin, typename T>
+hipError_t rows_one(const RowArgs& a, unsigned grid, hipStream_t s) {
+  if (a.Tb == 16) return rows_np<LOGX, Tin, T, 8>(a, grid, s);
+  if (a.Tb == 8) return rows_np<LOGX, Tin, T, 4>(a, grid, s);
+  return hipErrorInvalidValue;
+}
+
+template <typename Tin, typename T>
+hipError_t rows_logx(const RowArgs& a, unsigned grid, hipStream_t s) {
+  switch (a.logn) {
+    case 8: return rows_one<8, Tin, T>(a, grid, s);
+    case 9: return rows_one<9, Tin, T>(a, grid, s);
+    case 10: return rows_one<10, Tin, T>(a, grid, s);
+    case 11: return rows_one<11, Tin, T>(a, grid, s);
+    case 12: return rows_one<12, Tin, T>(a, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t rows_in(const RowArgs& a, unsigned grid, hipStream_t s) {
+  switch (a.in_dtype) {
+    case IN_U8: return rows_logx<uint8_t, T>(a, grid, s);
+    case IN_I16: return rows_logx<int16_t, T>(a, grid, s);
+    case IN_U16: return rows_logx<uint16_t, T>(a, grid, s);
+    case IN_I32: return rows_logx<int32_t, T>(a, grid, s);
+    case IN_U32: return rows_logx<uint32_t, T>(a, grid, s);
+    case IN_I64: return rows_logx<int64_t, T>(a, grid, s);
+    case IN_U64: return rows_logx<uint64_t, T>(a, grid, s);
+    case IN_F32: return rows_logx<float, T>(a, grid, s);
+    case IN_F64: return rows_logx<double, T>(a, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T, int LOGY, int TB>
+hipError_t cols_one(const ColArgs& a, unsigned grid, hipStream_t s) {
+  const size_t lds = sizeof(cplx_t<T>) * lds_padded_size((1 << LOGY) * TB);
+  auto kernel = k_cols_p2<T, LOGY, TB>;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), lds, s, a);
+  return hipGetLastError();
+}
+
+template <typename T, int TB>
+hipError_t cols_logy(const ColArgs& a, unsigned grid, hipStream_t s) {
+  switch (a.logn) {
+    case 8: return cols_one<T, 8, TB>(a, grid, s);
+    case 9: return cols_one<T, 9, TB>(a, grid, s);
+    case 10: return cols_one<T, 10, TB>(a, grid, s);
+    case 11: return cols_one<T, 11, TB>(a, grid, s);
+    case 12: return cols_one<T, 12, TB>(a, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t cols_tb(const ColArgs& a, unsigned grid, hipStream_t s) {
+  switch (a.Tb) {
+    case 4: return cols_logy<T, 4>(a, grid, s);
+    case 8: return cols_logy<T, 8>(a, grid, s);
+    case 16: return cols_logy<T, 16>(a, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+bool p2_spatial_ok(int logn) { return logn >= 8 && logn <= 12; }
+
+// Twiddles of the spatial power-of-two plan: passes 1 and 2 of sp_plan,
+// [k][r] = exp(-2 pi i r k / (NS R)).
+int p2_spatial_plan(int logn, int* r0, int* r1, int* r2) {
+  if (!p2_spatial_ok(logn)) return 0;
+  const SpPlan P = sp_plan(logn);
+  *r0 = P.r0;
+  *r1 = P.r1;
+  *r2 = P.r2;
+  return 1;
+}
+
+hipError_t launch_rows_p2(const RowArgs& a, int precision, hipStream_t s) {
+  const uint64_t nblocks = (a.frames + a.Tb - 1) / a.Tb;
+  const unsigned grid = static_cast<unsigned>(nblocks * a.H);
+  if (grid == 0) return hipSuccess;
+  return precision == PREC_FP64 ? rows_in<double>(a, grid, s) : rows_in<float>(a, grid, s);
+}
+
+hipError_t launch_cols_p2(const ColArgs& a, int precision, hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>(a.nblocks * a.nxh);
+  if (grid == 0) return hipSuccess;
+  return precision == PREC_FP64 ? cols_tb<double>(a, grid, s) : cols_tb<float>(a, grid, s);
+}
+
+}  // namespace fddm

@@ -23,191 +23,16 @@
 #include <type_traits>
 #include <utility>
 
-#include "fddm_fft.hpp"
 #include "fddm_kernels.hpp"
+#include "fddm_p2.hpp"
 
 namespace fddm {
 namespace {
+using namespace p2;
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kEdge = 32;
-
-// ---------------------------------------------------------------------------
-// Compile-time cos / sin of 2 pi a / b: exact octant reduction of the
-// rational angle, then Taylor series on [0, pi/4].
-// ---------------------------------------------------------------------------
-constexpr double kPi = 3.14159265358979323846264338327950288;
-constexpr double taylor_sin(double x) {
-  double x2 = x * x, term = x, sum = x;
-  for (int i = 1; i < 14; ++i) {
-    term *= -x2 / ((2.0 * i) * (2.0 * i + 1.0));
-    sum += term;
-  }
-  return sum;
-}
-constexpr double taylor_cos(double x) {
-  double x2 = x * x, term = 1.0, sum = 1.0;
-  for (int i = 1; i < 14; ++i) {
-    term *= -x2 / ((2.0 * i - 1.0) * (2.0 * i));
-    sum += term;
-  }
-  return sum;
-}
-struct CosSin {
-  double c, s;
-};
-constexpr CosSin cossin2pi(long long a, long long b) {
-  a %= b;
-  if (a < 0) a += b;
-  const long long o = (8 * a) / b;
-  const long long rem = 8 * a - o * b;
-  const double th = kPi / 4 * static_cast<double>(rem) / static_cast<double>(b);
-  const double ph = kPi / 4 * static_cast<double>(b - rem) / static_cast<double>(b);
-  const double ct = taylor_cos(th), st = taylor_sin(th), cp = taylor_cos(ph), sp = taylor_sin(ph);
-  switch (o) {
-    case 0: return {ct, st};
-    case 1: return {sp, cp};
-    case 2: return {-st, ct};
-    case 3: return {-cp, sp};
-    case 4: return {-ct, -st};
-    case 5: return {-sp, -cp};
-    case 6: return {st, -ct};
-    default: return {cp, -sp};
-  }
-}
-
-// a * exp(-2 pi i E / N), constant folded.
-template <typename T, int E, int N>
-__device__ __forceinline__ cplx_t<T> mul_w(cplx_t<T> a) {
-  using C = cplx_t<T>;
-  constexpr int e = ((E % N) + N) % N;
-  constexpr T h = static_cast<T>(0.70710678118654752440084436210485);
-  if constexpr (e == 0) {
-    return a;
-  } else if constexpr (4 * e == N) {
-    return C{a.y, -a.x};
-  } else if constexpr (2 * e == N) {
-    return C{-a.x, -a.y};
-  } else if constexpr (4 * e == 3 * N) {
-    return C{-a.y, a.x};
-  } else if constexpr (8 * e == N) {
-    return C{(a.x + a.y) * h, (a.y - a.x) * h};
-  } else if constexpr (8 * e == 3 * N) {
-    return C{(a.y - a.x) * h, -(a.x + a.y) * h};
-  } else if constexpr (8 * e == 5 * N) {
-    return C{-(a.x + a.y) * h, (a.x - a.y) * h};
-  } else if constexpr (8 * e == 7 * N) {
-    return C{(a.x - a.y) * h, (a.x + a.y) * h};
-  } else {
-    constexpr CosSin w = cossin2pi(e, N);
-    constexpr T c = static_cast<T>(w.c), s = static_cast<T>(w.s);
-    return C{a.x * c + a.y * s, a.y * c - a.x * s};
-  }
-}
-
-template <int I, int N, typename F>
-__device__ __forceinline__ void sfor(F&& f) {
-  if constexpr (I < N) {
-    f(std::integral_constant<int, I>{});
-    sfor<I + 1, N>(f);
-  }
-}
-
-template <typename T, int R>
-__device__ __forceinline__ void dft(cplx_t<T> (&v)[R]);
-
-// Length R1 R2 DFT in place, natural order in and out (Cooley-Tukey):
-// X[k1 + R1 k2] = sum_n2 W_R2^(n2 k2) W_N^(n2 k1) sum_n1 x[R2 n1 + n2] W_R1^(n1 k1).
-template <typename T, int R1, int R2>
-__device__ __forceinline__ void dft_composite(cplx_t<T> (&v)[R1 * R2]) {
-  using C = cplx_t<T>;
-  constexpr int N = R1 * R2;
-  C a[R2][R1];
-  sfor<0, R2>([&](auto n2c) {
-    constexpr int n2 = decltype(n2c)::value;
-#pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) a[n2][n1] = v[R2 * n1 + n2];
-    dft<T, R1>(a[n2]);
-    sfor<1, R1>([&](auto k1c) {
-      constexpr int k1 = decltype(k1c)::value;
-      a[n2][k1] = mul_w<T, n2 * k1, N>(a[n2][k1]);
-    });
-  });
-#pragma unroll
-  for (int k1 = 0; k1 < R1; ++k1) {
-    C b[R2];
-#pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) b[n2] = a[n2][k1];
-    dft<T, R2>(b);
-#pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
-  }
-}
-
-template <typename T, int R>
-__device__ __forceinline__ void dft(cplx_t<T> (&v)[R]) {
-  if constexpr (R <= 8) {
-    butterfly<T, R>(v);
-  } else if constexpr (R == 16) {
-    dft_composite<T, 4, 4>(v);
-  } else {
-    static_assert(R == 32, "radix");
-    dft_composite<T, 8, 4>(v);
-  }
-}
-
-__device__ __forceinline__ int padi(int e) { return e + (e >> 5); }
-constexpr int cpad(int e) { return e + (e >> 5); }
-
-// One Stockham pass with compile-time size: radix R after NS points, T
-// threads per transform, MB butterflies per thread.  Reads x[j + r M],
-// writes y[(j - k) R + k + r NS] (k = j mod NS); with M and NS powers of two
-// the padded addresses are pad(j) + cpad(r M) and pad((j - k) R + k) +
-// cpad(r NS), i.e. one base per butterfly plus immediate offsets.
-template <typename T, int N, int R, int NS, int TPT>
-struct Pass {
-  using C = cplx_t<T>;
-  static constexpr int M = N / R;
-  static constexpr int MB = M / TPT;
-  static_assert(MB >= 1 && M % TPT == 0, "whole butterflies per thread");
-  static_assert(M % 32 == 0, "linear padded reads");
-
-  static __device__ __forceinline__ void compute(const C* buf, C (&v)[MB][R], const C* __restrict__ tw, int u) {
-    // opaque thread index: the address arithmetic of each pass stays in that
-    // pass instead of being hoisted to the kernel entry and kept live
-    asm volatile("" : "+v"(u));
-#pragma unroll
-    for (int b = 0; b < MB; ++b) {
-      const int j = u + b * TPT;
-      const C* p = buf + padi(j);
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = p[cpad(r * M)];
-      if constexpr (NS > 1) {
-        const int k = j & (NS - 1);
-        const C* t = tw + k * R;
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], t[r]);
-      }
-      dft<T, R>(v[b]);
-    }
-  }
-  static __device__ __forceinline__ void compute_regs(C (&v)[MB][R]) {
-#pragma unroll
-    for (int b = 0; b < MB; ++b) dft<T, R>(v[b]);
-  }
-  static __device__ __forceinline__ void store(C* buf, const C (&v)[MB][R], int u) {
-    asm volatile("" : "+v"(u));
-#pragma unroll
-    for (int b = 0; b < MB; ++b) {
-      const int j = u + b * TPT;
-      const int k = j & (NS - 1);
-      C* p = buf + padi((j - k) * R + k);
-#pragma unroll
-      for (int r = 0; r < R; ++r) p[cpad(r * NS)] = v[b][r];
-    }
-  }
-};
 
 template <typename A>
 __device__ __forceinline__ A wave_sum(A v) {
@@ -382,11 +207,18 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
       const int t = u + b * TPT + r * M0;
       if (t < L) buf[padi(t)].x = v0[b][r].x;
     }
-  Acc ptot = 0;
-  for (int c0 = 0; c0 < L; c0 += TPT) {
-    const int t = c0 + u;
+  // each thread: a contiguous segment of seg <= SEGMAX samples, local running
+  // sums, then one exclusive scan over the group's threads
+  constexpr int SEGMAX = (N / 2 + TPT - 1) / TPT;
+  const int seg = (L + TPT - 1) / TPT;
+  const int ts = u * seg;
+  Acc pre[SEGMAX];
+  Acc run = 0;
+#pragma unroll
+  for (int i = 0; i < SEGMAX; ++i) {
+    const int t = ts + i;
     Acc val = 0;
-    if (active && t < L) {
+    if (i < seg && active && t < L) {
       C f = src[t];
       f.x -= m_r;
       f.y -= m_i;
@@ -396,21 +228,27 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
       }
       val = static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
     }
-    Acc inc = wave_incl_scan(val);
-    if (lane == 63) chunk[wg] = inc;
-    __syncthreads();
-    Acc pre = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < WPT; ++w) {
-      const Acc x = chunk[w];
-      tot += x;
-      if (w < wg) pre += x;
-    }
-    inc += ptot + pre;
-    if (t < L - 1) tail[t] = inc;
-    ptot += tot;
-    __syncthreads();
+    run += val;
+    pre[i] = run;
   }
+  const Acc inc = wave_incl_scan(run);
+  Acc exc = __shfl_up(inc, 1, 64);
+  if (lane == 0) exc = 0;
+  if (lane == 63) chunk[wg] = inc;
+  __syncthreads();
+  Acc ptot = 0;
+#pragma unroll
+  for (int w = 0; w < WPT; ++w) {
+    const Acc x = chunk[w];
+    if (w < wg) exc += x;
+    ptot += x;
+  }
+#pragma unroll
+  for (int i = 0; i < SEGMAX; ++i) {
+    const int t = ts + i;
+    if (i < seg && t < L - 1) tail[t] = pre[i] + exc;
+  }
+  __syncthreads();
   if (!active) return;
 
   OutT* __restrict__ out = static_cast<OutT*>(a.out);

@@ -76,6 +76,11 @@ def test_fft_engine_matches_numpy(hip, precision, tol):
     ((21, 30, 40), 40, 30, False),
     ((18, 35, 21), 21, 35, True),
     ((9, 30, 40), 42, 32, True),
+    # power-of-two kernels (256 .. 4096): exact sizes and zero padding
+    ((6, 256, 256), 256, 256, True),
+    ((5, 255, 509), 512, 256, True),
+    ((4, 512, 1024), 1024, 512, False),
+    ((3, 1024, 256), 256, 1024, True),
 ])
 def test_spatial_spectra_match_oracle(hip, precision, shape, nx, ny, window):
     L, H, W = shape
@@ -101,6 +106,8 @@ CASES = [
     (25, 21, 27, 27, 21, False, [1, 2, 3, 10, 24]),
     (16, 24, 20, 21, 25, True, [1, 5, 15]),  # zero padding in x and y
     (64, 20, 48, 48, 20, False, "all"),
+    (24, 256, 256, 256, 256, True, "all"),   # power-of-two spatial kernels
+    (19, 250, 255, 256, 256, False, [1, 2, 9, 18]),
 ]
 
 
