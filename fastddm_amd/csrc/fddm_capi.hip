@@ -273,9 +273,10 @@ int plan_create(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H, ui
                         (unsigned long long)nx, (unsigned long long)ny));
   if (p->nq > (1ull << 31)) return fail(set_err(err, errlen, FDDM_ERR_SIZE, "image too large"));
   // Frames per block: runs of Tb * sizeof(complex) = 128 B in the q-major
-  // store, halved while a column slab (ny x Tb complex) exceeds 64 KiB.
+  // store, halved while a column slab (ny x Tb complex) exceeds 80 KiB (two
+  // slabs per CU).
   p->Tb = f64 ? 8 : 16;
-  while (p->Tb > 2 && lds_padded_size(static_cast<int>(ny) * p->Tb) * csz > 64 * 1024) p->Tb /= 2;
+  while (p->Tb > 2 && lds_padded_size(static_cast<int>(ny) * p->Tb) * csz > 80 * 1024) p->Tb /= 2;
   if (lds_padded_size(static_cast<int>(ny) * p->Tb) * csz > kLdsLimit)
     return fail(set_err(err, errlen, FDDM_ERR_SIZE, "ny=%llu too large for the column pass", (unsigned long long)ny));
   // Transforms handled at once by the row / column kernels (<= 16 or 32

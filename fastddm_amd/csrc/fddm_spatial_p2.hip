@@ -8,7 +8,9 @@
 //              store R[tb][kx][y][ti] with the frame index fastest.
 //   k_cols_p2  one kx column of Tb frames: length-NY FFTs of the interleaved
 //              slab [y][ti]; the last pass writes the q-major spectra
-//              S[ky * nxh + kx][t] straight from registers.
+//              S[ky * nxh + kx][t] straight from registers.  Slabs above
+//              64 KiB leave room for two workgroups per CU, which then run
+//              1024 threads each to keep 32 waves resident.
 #include "fddm_kernels.hpp"
 #include "fddm_p2.hpp"
 
@@ -103,14 +105,14 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a) {
   }
 }
 
-template <typename T, int LOGY, int TB>
-__global__ void __launch_bounds__(kThreads) k_cols_p2(ColArgs a) {
+template <typename T, int LOGY, int TB, int NT>
+__global__ void __launch_bounds__(NT) k_cols_p2(ColArgs a) {
   using C = cplx_t<T>;
   constexpr int NY = 1 << LOGY;
   constexpr SpPlan P = sp_plan(LOGY);
-  using A0 = Pass<T, NY, P.r0, 1, kThreads, TB, true, TB, 1>;
-  using A1 = Pass<T, NY, P.r1, P.r0, kThreads, TB, true, TB, 1>;
-  using A2 = Pass<T, NY, P.r2, P.r0 * P.r1, kThreads, TB, true, TB, 1>;
+  using A0 = Pass<T, NY, P.r0, 1, NT, TB, true, TB, 1>;
+  using A1 = Pass<T, NY, P.r1, P.r0, NT, TB, true, TB, 1>;
+  using A2 = Pass<T, NY, P.r2, P.r0 * P.r1, NT, TB, true, TB, 1>;
   extern __shared__ __align__(16) unsigned char smem[];
   C* const buf = reinterpret_cast<C*>(smem);
   const int tid = threadIdx.x;
@@ -119,7 +121,7 @@ __global__ void __launch_bounds__(kThreads) k_cols_p2(ColArgs a) {
   const uint64_t tb = blockIdx.x / nxh;
   const C* __restrict__ src = static_cast<const C*>(a.R) + (tb * nxh + kx) * static_cast<size_t>(H) * TB;
   const int nin = H * TB;
-  for (int e = tid; e < NY * TB; e += kThreads) buf[padi(e)] = e < nin ? src[e] : C{0, 0};
+  for (int e = tid; e < NY * TB; e += NT) buf[padi(e)] = e < nin ? src[e] : C{0, 0};
   __syncthreads();
   const C* __restrict__ tw = static_cast<const C*>(a.twy);
   {
@@ -143,8 +145,8 @@ __global__ void __launch_bounds__(kThreads) k_cols_p2(ColArgs a) {
   constexpr int M2 = NY / P.r2;
 #pragma unroll
   for (int b = 0; b < A2::MB; ++b) {
-    const int w = tid + b * kThreads;
-    if (A2::ITEMS % kThreads != 0 && w >= A2::ITEMS) break;
+    const int w = tid + b * NT;
+    if (A2::ITEMS % NT != 0 && w >= A2::ITEMS) break;
     const int ti = w % TB, j = w / TB;
 #pragma unroll
     for (int r = 0; r < P.r2; ++r) {
@@ -204,14 +206,15 @@ hipError_t rows_in(const RowArgs& a, unsigned grid, hipStream_t s) {
 
 template <typename T, int LOGY, int TB>
 hipError_t cols_one(const ColArgs& a, unsigned grid, hipStream_t s) {
-  const size_t lds = sizeof(cplx_t<T>) * lds_padded_size((1 << LOGY) * TB);
-  auto kernel = k_cols_p2<T, LOGY, TB>;
+  constexpr size_t lds = sizeof(cplx_t<T>) * lds_padded_size((1 << LOGY) * TB);
+  constexpr int NT = lds > 64 * 1024 ? 1024 : kThreads;
+  auto kernel = k_cols_p2<T, LOGY, TB, NT>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(NT), lds, s, a);
   return hipGetLastError();
 }
 

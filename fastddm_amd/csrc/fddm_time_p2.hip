@@ -52,19 +52,27 @@ __device__ __forceinline__ A wave_incl_scan(A v) {
   return v;
 }
 
-// Sum over the WPT waves of this thread's group; red holds kWaves doubles.
+// Sums of x and y over the WPT waves of this thread's group (one barrier);
+// red holds 2 kWaves doubles and is written once per launch.
 template <int WPT>
-__device__ __forceinline__ double group_sum(double v, double* red) {
+__device__ __forceinline__ void group_sum2(double& x, double& y, double* red) {
   const int wave = threadIdx.x >> 6;
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) red[wave] = v;
+  x = wave_sum(x);
+  y = wave_sum(y);
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * wave] = x;
+    red[2 * wave + 1] = y;
+  }
   __syncthreads();
   const int first = (wave / WPT) * WPT;
-  double s = 0;
+  double sx = 0, sy = 0;
 #pragma unroll
-  for (int w = 0; w < WPT; ++w) s += red[first + w];
-  __syncthreads();
-  return s;
+  for (int w = 0; w < WPT; ++w) {
+    sx += red[2 * (first + w)];
+    sy += red[2 * (first + w) + 1];
+  }
+  x = sx;
+  y = sy;
 }
 
 // Twiddle table offsets (complex entries): FFT1 passes 1, 2 then FFT2 passes 1, 2;
@@ -90,7 +98,7 @@ struct P2Lds {
   static constexpr size_t csz = sizeof(cplx_t<T>);
   static constexpr size_t buf = static_cast<size_t>(G) * lds_padded_size(N) * csz;
   static constexpr size_t edge = static_cast<size_t>(G) * 2 * kEdge * csz;
-  static constexpr size_t red = kWaves * sizeof(double);
+  static constexpr size_t red = 2 * kWaves * sizeof(double);
   static constexpr size_t chunk = static_cast<size_t>(G) * WPT * sizeof(Acc);
   static constexpr size_t bytes = buf + edge + red + chunk;
 };
@@ -148,8 +156,7 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
       sr += static_cast<double>(v0[b][r].x);
       si += static_cast<double>(v0[b][r].y);
     }
-  sr = group_sum<WPT>(sr, red);
-  si = group_sum<WPT>(si, red);
+  group_sum2<WPT>(sr, si, red);
   const double mr = sr / L, mi = si / L;
   const T m_r = SUB ? static_cast<T>(mr) : T(0), m_i = SUB ? static_cast<T>(mi) : T(0);
   if constexpr (SUB) {
@@ -208,9 +215,10 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
       if (t < L) buf[padi(t)].x = v0[b][r].x;
     }
   // each thread: a contiguous segment of seg <= SEGMAX samples, local running
-  // sums, then one exclusive scan over the group's threads
-  constexpr int SEGMAX = (N / 2 + TPT - 1) / TPT;
-  const int seg = (L + TPT - 1) / TPT;
+  // sums, then one exclusive scan over the group's threads; seg is odd, so
+  // the 8-byte stores of neighbouring threads fall in different LDS banks
+  constexpr int SEGMAX = ((N / 2 + TPT - 1) / TPT) | 1;
+  const int seg = ((L + TPT - 1) / TPT) | 1;
   const int ts = u * seg;
   Acc pre[SEGMAX];
   Acc run = 0;
