@@ -45,9 +45,9 @@ def algorithmic_bytes(L, H, W, b_in, nq, nl, c, o, window):
     return {"spatial": spatial, "time": time_stage, "whole": spatial + time_stage}
 
 
-def run_gpu(core, img, lags, nx, ny, nt, win, precision, steps, warmup, keep_output):
+def run_gpu(core, img, lags, nx, ny, nt, win, precision, steps, warmup, keep_output, mode="fft"):
     L, H, W = img.shape
-    plan = core.Plan("fft", img.dtype, img.shape, lags, nx, ny, nt, win, precision, np.float64)
+    plan = core.Plan(mode, img.dtype, img.shape, lags, nx, ny, nt, win, precision, np.float64)
     d_img = core.DeviceBuffer.from_array(img)
     d_out = core.DeviceBuffer(plan.output_size())
     d_ws = core.DeviceBuffer(plan.workspace_size())
@@ -111,8 +111,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--frames", type=int, default=2000)
-    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--workload", choices=["fft", "diff"], default="fft",
+                    help="fft: 2000 x 512^2 uint16, all lags (default); diff: 300 x 256^2 uint8, "
+                         "50 log-spaced lags, no window")
+    ap.add_argument("--frames", type=int, default=None)
+    ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -127,6 +130,7 @@ def main():
     from fastddm_amd import _core_hip as core
     from fastddm_amd import next_fast_len
     from fastddm_amd.speckle import ar1_speckle
+    from fastddm_amd.lags import logspace_int
     from fastddm_amd.window import blackman_harris
 
     if core.get_num_devices() < 1:
@@ -140,26 +144,47 @@ def main():
         log("copied 3 x 1 GiB device to device")
         return
 
-    L, S = args.frames, args.size
+    diff = args.workload == "diff"
+    L = args.frames or (300 if diff else 2000)
+    S = args.size or (256 if diff else 512)
     t0 = time.perf_counter()
-    img = ar1_speckle(L, S, S, sigma=1.5, diffusivity=0.2, mean=1000.0, std=150.0,
-                      dtype=np.uint16, seed=2)
+    if diff:
+        img = ar1_speckle(L, S, S, sigma=2.0, diffusivity=1.0, mean=120.0, std=25.0,
+                          dtype=np.uint8, seed=3)
+        win = None
+        lags = logspace_int(L - 1, num=50, endpoint=True)
+    else:
+        img = ar1_speckle(L, S, S, sigma=1.5, diffusivity=0.2, mean=1000.0, std=150.0,
+                          dtype=np.uint16, seed=2)
+        win = blackman_harris((S, S))
+        lags = np.arange(1, L)
     log(f"generated {img.shape} {img.dtype} speckle in {time.perf_counter() - t0:.1f} s")
-    win = blackman_harris((S, S))
-    lags = np.arange(1, L)
     nx = next_fast_len(S, "hip")
     ny = nx
     nt = next_fast_len(2 * L, "hip")
     nxh = nx // 2 + 1
     nq = ny * nxh
 
+    mode = "diff" if diff else "fft"
     ms32, st32, out32 = run_gpu(core, img, lags, nx, ny, nt, win, "fp32c", args.steps, args.warmup,
-                                keep_output=not args.skip_fp64)
+                                keep_output=not args.skip_fp64, mode=mode)
     log(f"fp32c: {ms32:.3f} ms/step, stages {st32}")
+    # host numpy in -> host numpy out through the C ABI (PCIe both ways, device
+    # buffers cached after the first call); reported beside, never as value
+    e2e = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        if diff:
+            core.ddm_diff(img, lags, nx, ny, win, precision="fp32c")
+        else:
+            core.ddm_fft(img, lags, nx, ny, nt, win, precision="fp32c")
+        e2e.append(time.perf_counter() - t0)
+    e2e_s = min(e2e[1:])
+    log(f"host-to-host fp32c: {e2e_s * 1e3:.1f} ms per call")
     fp64 = None
     if not args.skip_fp64:
         ms64, st64, out64 = run_gpu(core, img, lags, nx, ny, nt, win, "fp64", args.steps,
-                                    args.warmup, keep_output=True)
+                                    args.warmup, keep_output=True, mode=mode)
         log(f"fp64: {ms64:.3f} ms/step, stages {st64}")
         d32, d64 = out32[:-2].ravel(), out64[:-2].ravel()
         m = d64 != 0
@@ -172,9 +197,9 @@ def main():
         }
         del out32, out64, d32, d64, rel
 
-    b = algorithmic_bytes(L, S, S, img.itemsize, nq, len(lags), 8, 8, True)
+    b = algorithmic_bytes(L, S, S, img.itemsize, nq, len(lags), 8, 8, win is not None)
     kernels = {
-        "time_wk": (st32["time_or_diff"], b["time"]),
+        ("diff" if diff else "time_wk"): (st32["time_or_diff"], b["time"]),
         "spatial_rows_cols": (st32["row_pass"] + st32["column_pass"], b["spatial"]),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
@@ -185,7 +210,7 @@ def main():
     if os.path.isfile(tf):
         with open(tf) as f:
             t = json.load(f)
-        entry = t.get("k_time_fp32c_512x2000") if dom == "time_wk" else None
+        entry = t.get("k_time_fp32c_512x2000") if (dom == "time_wk" and (L, S) == (2000, 512)) else None
         if entry:
             traffic = entry.get("bytes_per_launch")
 
@@ -201,9 +226,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic AR(1) Brownian speckle, uint16, seed 2 (device-resident input)",
+        "data": (f"synthetic AR(1) Brownian speckle, {img.dtype}, seed {3 if diff else 2} "
+                 "(device-resident input)"),
         "config": {
-            "workload": (f"fft mode, {L} frames of {S}x{S} uint16, lags 1..{L - 1}, Blackman-Harris "
+            "workload": (f"diff mode, {L} frames of {S}x{S} uint8, {len(lags)} log-spaced lags in "
+                         f"1..{L - 1}, no window, precision fp32c, float64 output" if diff else
+                         f"fft mode, {L} frames of {S}x{S} uint16, lags 1..{L - 1}, Blackman-Harris "
                          f"window, nt={nt}, precision fp32c, float64 output"),
             "frames": L, "height": S, "width": S, "nt": nt, "lags": len(lags),
             "precision": "fp32c", "parallelism": "single GPU",
@@ -219,6 +247,12 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes": dom_bytes,
         },
+        "host_to_host": {
+            "frames_per_s": L / e2e_s,
+            "ms_per_call": e2e_s * 1e3,
+            "note": "ddm_fft through the C ABI from a host numpy stack to a host numpy result "
+                    "(H2D of the uint16 frames and D2H of the float64 output included)",
+        },
         "whole_path": {
             "algorithmic_bytes": b["whole"],
             "effective_GBs": b["whole"] / (ms32 / 1e3) / 1e9,
@@ -228,7 +262,7 @@ def main():
     if fp64 is not None:
         result["fp64"] = fp64
         result["speedup_vs_fp64"] = fp64["ms_per_step"] / ms32
-    if not args.skip_cpu:
+    if not args.skip_cpu and not diff:
         t0 = time.perf_counter()
         result["cpu_baseline"] = cpu_baseline(img, lags, nx, ny, nt, win, L, nq)
         log(f"cpu baseline sample took {time.perf_counter() - t0:.1f} s")

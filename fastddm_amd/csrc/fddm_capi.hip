@@ -149,11 +149,11 @@ std::vector<T> twiddles(const FftDesc& d) {
   return w;
 }
 
-// Twiddles of the power-of-two time plan: FFT1 passes 1, 2 then FFT2 passes
-// 1, 2 (p2_plan radices), each [k][r] = exp(-2 pi i r k / (NS R)).
+// Twiddles of the power-of-two time plan (p2_plan): FFT1 passes 1 .. np-1,
+// then FFT2 passes 1 .. np-1, each [k][r] = exp(-2 pi i r k / (NS R)).
 template <typename T>
 std::vector<T> p2_twiddles(int logn) {
-  const P2Plan P = p2_plan(logn);
+  const P2Plan P = p2_plan(logn, sizeof(T) == 8);
   std::vector<T> w;
   const long double two_pi = 6.283185307179586476925286766559005768L;
   auto pass = [&](long long R, long long NS) {
@@ -164,10 +164,8 @@ std::vector<T> p2_twiddles(int logn) {
         w.push_back(static_cast<T>(sinl(ang)));
       }
   };
-  pass(P.r1, P.r0);
-  pass(P.r2, static_cast<long long>(P.r0) * P.r1);
-  pass(P.r1, P.r2);
-  pass(P.r0, static_cast<long long>(P.r2) * P.r1);
+  for (int i = 1; i < P.np; ++i) pass(p2_radix(P, i), p2_ns(P, i));
+  for (int i = 1; i < P.np; ++i) pass(p2_radix_b(P, i), p2_ns_b(P, i));
   return w;
 }
 

@@ -47,17 +47,43 @@ __host__ __device__ constexpr int lds_padded_size(int n) { return n + (n >> 5) +
 // transform, three passes of radices R0, R1, R2 (forward order; the second
 // transform runs them in reverse).  Every pass has a whole number of
 // butterflies per thread: N / (R * T) >= 1.
+// Time-axis plans for N = 2^logn: threads per transform, number of passes
+// (3 or 4) and the radices of FFT1 in order (FFT2 runs them reversed).  For
+// FP32 at N >= 4096 the transform is spread over 512 or 1024 threads (radix
+// 8 or 16 passes) so that a CU holds 32 (N = 4096) or 16 waves despite the
+// LDS buffer; FP64 keeps three passes on 256 threads, which measured faster
+// than the four-pass plans at its register and LDS footprint.
 struct P2Plan {
-  int threads, r0, r1, r2;
+  int threads, np, r0, r1, r2, r3;
 };
-__host__ __device__ constexpr P2Plan p2_plan(int logn) {
-  return logn == 9    ? P2Plan{64, 8, 8, 8}
-         : logn == 10 ? P2Plan{64, 16, 16, 4}
-         : logn == 11 ? P2Plan{128, 16, 16, 8}
-         : logn == 12 ? P2Plan{256, 16, 16, 16}
-         : logn == 13 ? P2Plan{256, 16, 16, 32}
-                      : P2Plan{256, 16, 32, 32};
+__host__ __device__ constexpr P2Plan p2_plan(int logn, bool f64) {
+  return logn == 9    ? P2Plan{64, 3, 8, 8, 8, 1}
+         : logn == 10 ? P2Plan{64, 3, 16, 16, 4, 1}
+         : logn == 11 ? P2Plan{128, 3, 16, 16, 8, 1}
+         : logn == 12 ? (f64 ? P2Plan{256, 3, 16, 16, 16, 1} : P2Plan{512, 4, 8, 8, 8, 8})
+         : logn == 13 ? (f64 ? P2Plan{256, 3, 16, 16, 32, 1} : P2Plan{512, 4, 16, 8, 8, 8})
+                      : P2Plan{1024, 4, 16, 8, 8, 16};
 }
+__host__ __device__ constexpr int p2_radix(const P2Plan& P, int i) {
+  return i == 0 ? P.r0 : i == 1 ? P.r1 : i == 2 ? P.r2 : P.r3;
+}
+// Radix of pass i of FFT2 (the FFT1 radices reversed).
+__host__ __device__ constexpr int p2_radix_b(const P2Plan& P, int i) { return p2_radix(P, P.np - 1 - i); }
+// Points already combined before pass i (product of the earlier radices).
+__host__ __device__ constexpr int p2_ns(const P2Plan& P, int i) { return i == 0 ? 1 : p2_ns(P, i - 1) * p2_radix(P, i - 1); }
+__host__ __device__ constexpr int p2_ns_b(const P2Plan& P, int i) {
+  return i == 0 ? 1 : p2_ns_b(P, i - 1) * p2_radix_b(P, i - 1);
+}
+// Twiddle table layout (complex entries): FFT1 passes 1 .. np-1, then FFT2
+// passes 1 .. np-1; the pass with radix R after NS points holds
+// exp(-2 pi i r k / (NS R)) at offset k R + r of its table (NS R entries).
+__host__ __device__ constexpr int p2_tw_a(const P2Plan& P, int i) {
+  return i <= 1 ? 0 : p2_tw_a(P, i - 1) + p2_ns(P, i - 1) * p2_radix(P, i - 1);
+}
+__host__ __device__ constexpr int p2_tw_b(const P2Plan& P, int i) {
+  return i <= 1 ? p2_tw_a(P, P.np) : p2_tw_b(P, i - 1) + p2_ns_b(P, i - 1) * p2_radix_b(P, i - 1);
+}
+__host__ __device__ constexpr int p2_tw_total(const P2Plan& P) { return p2_tw_b(P, P.np); }
 constexpr int kP2MinLog = 9, kP2MaxLog = 14;
 
 template <typename T> struct Cplx;

@@ -6,10 +6,9 @@
 //   k_cols  stage (a) column pass + stage (b) transpose: length-ny FFT of Tb
 //           frames of one kx column at once, store q-major S[q][t] in runs of
 //           Tb consecutive frames.
-//   k_time  stage (c) + (e): per-q Wiener-Khinchin correlation (zero-padded
-//           length-nt FFT, |.|^2, second forward FFT), prefix sums of |F|^2,
-//           D(q, dt) for the requested lags, PS and VAR, y-shifted store.
 //   k_diff  stage (d) + (e): per-q direct lag differences, PS and VAR.
+//
+// The fft-mode time stage lives in fddm_time_p2.hip (power-of-two lengths).
 //
 // Semantics follow the reference cores: spectra and normalisation
 // src/cpp/ddm.cc:288-323, correlation and two-ended sums ddm.cc:344-426,
@@ -23,7 +22,6 @@ namespace fddm {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-constexpr int kEdge = 32;  // lags with at most this many pairs use direct sums
 
 int fft_bucket(long long work) {
   const long long e = (work + kThreads - 1) / kThreads;
@@ -46,17 +44,6 @@ __device__ __forceinline__ A wave_sum(A v) {
 }
 
 template <typename A>
-__device__ __forceinline__ A wave_incl_scan(A v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const A u = __shfl_up(v, o, 64);
-    if (lane >= o) v += u;
-  }
-  return v;
-}
-
-template <typename A>
 __device__ __forceinline__ A block_sum(A v) {
   __shared__ A sh[kWaves];
   const A w = wave_sum(v);
@@ -67,28 +54,6 @@ __device__ __forceinline__ A block_sum(A v) {
   for (int i = 0; i < kWaves; ++i) tot += sh[i];
   __syncthreads();
   return tot;
-}
-
-// Exclusive block scan; `total` receives the sum over the block.
-template <typename A>
-__device__ __forceinline__ A block_excl_scan(A v, A& total) {
-  __shared__ A sh[kWaves];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const A inc = wave_incl_scan(v);
-  A exc = __shfl_up(inc, 1, 64);
-  if (lane == 0) exc = 0;
-  if (lane == 63) sh[wid] = inc;
-  __syncthreads();
-  A off = 0, tot = 0;
-#pragma unroll
-  for (int i = 0; i < kWaves; ++i) {
-    const A s = sh[i];
-    if (i < wid) off += s;
-    tot += s;
-  }
-  __syncthreads();
-  total = tot;
-  return off + exc;
 }
 
 // ---------------------------------------------------------------------------
@@ -211,141 +176,6 @@ __device__ __forceinline__ size_t shifted_q(int q, int ny, int nxh) {
   const int ky = q / nxh, kx = q - ky * nxh;
   const int oy = (ky + ny / 2) % ny;
   return static_cast<size_t>(oy) * nxh + kx;
-}
-
-// ---------------------------------------------------------------------------
-// Stage (c)+(e): Wiener-Khinchin per q.
-// ---------------------------------------------------------------------------
-template <typename T, typename OutT, int E, int PREC>
-__global__ void __launch_bounds__(kThreads) k_time(TimeArgs a, int q_per_xcd, int nq) {
-  using C = cplx_t<T>;
-  using Acc = std::conditional_t<PREC == PREC_FP32, float, double>;
-  constexpr bool kSub = PREC != PREC_FP32;
-  constexpr int SEG = E / 2 > 0 ? E / 2 : 1;
-  extern __shared__ __align__(16) unsigned char smem[];
-  C* buf = reinterpret_cast<C*>(smem);
-  const int tid = threadIdx.x;
-  // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
-  // contiguous range of q so neighbouring output columns share its L2.
-  const int q = (blockIdx.x & 7) * q_per_xcd + (blockIdx.x >> 3);
-  if (q >= nq) return;
-  const int L = a.L, n = a.dt.n;
-  const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(q) * a.Lp;
-  for (int t = tid; t < n; t += kThreads) buf[lds_pad(t)] = t < L ? src[t] : C{0, 0};
-  __syncthreads();
-
-  const int seg = (L + kThreads - 1) / kThreads;
-  const int t0 = min(L, tid * seg), t1 = min(L, t0 + seg);
-  double mr = 0.0, mi = 0.0;
-  if constexpr (kSub) {
-    for (int t = t0; t < t1; ++t) {
-      const C f = buf[lds_pad(t)];
-      mr += static_cast<double>(f.x);
-      mi += static_cast<double>(f.y);
-    }
-    mr = block_sum(mr) / L;
-    mi = block_sum(mi) / L;
-  }
-  const T m_r = static_cast<T>(mr), m_i = static_cast<T>(mi);
-  Acc pre[SEG];
-  Acc run = 0, sr = 0, si = 0;
-#pragma unroll
-  for (int i = 0; i < SEG; ++i) {
-    const int t = t0 + i;
-    if (t < t1) {
-      C f = buf[lds_pad(t)];
-      if constexpr (kSub) {
-        f.x -= m_r;
-        f.y -= m_i;
-        buf[lds_pad(t)] = f;
-      }
-      run += static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
-      sr += static_cast<Acc>(f.x);
-      si += static_cast<Acc>(f.y);
-      pre[i] = run;
-    }
-  }
-  Acc ptot;
-  const Acc off = block_excl_scan(run, ptot);
-  sr = block_sum(sr);
-  si = block_sum(si);
-  // Lags with few pairs (L - dt <= kEdge) are summed directly from the first
-  // and last kEdge samples: there the correlation form cancels badly (D is a
-  // sum of a handful of terms, the prefix sums are not).
-  __shared__ C edge[2 * kEdge];
-  if constexpr (kSub) {
-    for (int u = tid; u < kEdge; u += kThreads) {
-      edge[u] = u < L ? buf[lds_pad(u)] : C{0, 0};
-      const int s = L - kEdge + u;
-      edge[kEdge + u] = s >= 0 ? buf[lds_pad(s)] : C{0, 0};
-    }
-  }
-
-  const C* __restrict__ tw = static_cast<const C*>(a.twt);
-  fft_lds<T, E, false>(buf, 0, a.dt, 1, 0, n, 1, tw, tid, kThreads);
-  for (int t = tid; t < n; t += kThreads) {
-    const C g = buf[lds_pad(t)];
-    buf[lds_pad(t)] = C{g.x * g.x + g.y * g.y, 0};
-  }
-  __syncthreads();
-  fft_lds<T, E, false>(buf, 0, a.dt, 1, 0, n, 1, tw, tid, kThreads);
-  // Re buf[k] = n * C(k), C(k) = Re sum_t conj F(t) F(t + k).  Prefix sums
-  // psum(k) = sum_{t<k} |F(t)|^2, k = 1 .. L-1, go to the free tail of the
-  // buffer (elements L .. n-1 are no longer needed).
-  Acc* tail = reinterpret_cast<Acc*>(buf + lds_pad(L));
-#pragma unroll
-  for (int i = 0; i < SEG; ++i) {
-    const int t = t0 + i;
-    if (t < t1 && t < L - 1) tail[t] = pre[i] + off;
-  }
-  __syncthreads();
-
-  OutT* __restrict__ out = static_cast<OutT*>(a.out);
-  const size_t nqs = static_cast<size_t>(a.ny) * a.nxh;
-  const size_t oq = shifted_q(q, a.ny, a.nxh);
-  const Acc inv_n = static_cast<Acc>(1) / static_cast<Acc>(n);
-  for (int i = tid; i < a.nl; i += kThreads) {
-    const int d = static_cast<int>(a.lags[i]);
-    Acc D;
-    if (kSub && L - d <= kEdge) {
-      // samples s < kEdge are in edge[s], s >= L - kEdge in edge[s - L + 2 kEdge]
-      Acc acc = 0;
-      for (int t = 0; t < L - d; ++t) {
-        const int s = t + d;
-        const C u = s < kEdge ? edge[s] : edge[s - L + 2 * kEdge];
-        const C v = edge[t];
-        const Acc dr = static_cast<Acc>(u.x) - static_cast<Acc>(v.x);
-        const Acc di = static_cast<Acc>(u.y) - static_cast<Acc>(v.y);
-        acc += dr * dr + di * di;
-      }
-      D = acc / static_cast<Acc>(L - d);
-    } else {
-      const Acc head = tail[L - d - 1];   // psum(L - d)
-      const Acc tl = ptot - tail[d - 1];  // psum(L) - psum(d)
-      const Acc c = static_cast<Acc>(buf[lds_pad(d)].x) * inv_n;
-      D = (head + tl - 2 * c) / static_cast<Acc>(L - d);
-    }
-    out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(D);
-  }
-  if (tid == 0) {
-    const Acc mr2 = sr / L, mi2 = si / L;
-    Acc ps, var;
-    if constexpr (kSub) {
-      var = ptot / L - (mr2 * mr2 + mi2 * mi2);
-      Acc fr = static_cast<Acc>(m_r) + mr2, fi = static_cast<Acc>(m_i) + mi2;
-      if (a.S0) {
-        const C z = static_cast<const C*>(a.S0)[static_cast<size_t>(q) * a.s0_stride];
-        fr += static_cast<Acc>(z.x);
-        fi += static_cast<Acc>(z.y);
-      }
-      ps = var + fr * fr + fi * fi;
-    } else {
-      ps = ptot / L;
-      var = ps - (mr2 * mr2 + mi2 * mi2);
-    }
-    out[static_cast<size_t>(a.nl) * nqs + oq] = static_cast<OutT>(ps);
-    out[static_cast<size_t>(a.nl + 1) * nqs + oq] = static_cast<OutT>(var);
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -504,24 +334,6 @@ hipError_t launch_cols(const ColArgs& a, int precision, hipStream_t s) {
 
 namespace {
 template <typename T, typename OutT, int PREC>
-hipError_t time_e(const TimeArgs& a, int e, hipStream_t s) {
-  const int nq = a.ny * a.nxh;
-  const int qpx = (nq + 7) / 8;
-  const unsigned grid = static_cast<unsigned>(qpx) * 8u;
-  const size_t lds = sizeof(cplx_t<T>) * lds_padded_size(a.dt.n);
-  switch (e) {
-    case 4: return launch(k_time<T, OutT, 4, PREC>, grid, lds, s, a, qpx, nq);
-    case 8: return launch(k_time<T, OutT, 8, PREC>, grid, lds, s, a, qpx, nq);
-    case 16: return launch(k_time<T, OutT, 16, PREC>, grid, lds, s, a, qpx, nq);
-    case 32: return launch(k_time<T, OutT, 32, PREC>, grid, lds, s, a, qpx, nq);
-    case 64:
-      if constexpr (sizeof(T) == 4) return launch(k_time<T, OutT, 64, PREC>, grid, lds, s, a, qpx, nq);
-      return hipErrorInvalidValue;
-    default: return hipErrorInvalidValue;
-  }
-}
-
-template <typename T, typename OutT, int PREC>
 hipError_t diff_l(const TimeArgs& a, hipStream_t s) {
   const int nq = a.ny * a.nxh;
   const int qpx = (nq + 7) / 8;
@@ -531,20 +343,6 @@ hipError_t diff_l(const TimeArgs& a, hipStream_t s) {
   return launch(k_diff<T, OutT, PREC, false>, grid, 0, s, a, qpx, nq);
 }
 }  // namespace
-
-hipError_t launch_time(const TimeArgs& a, int precision, int out_dtype, hipStream_t s) {
-  const int e = fft_bucket(a.dt.n);
-  const bool f32 = out_dtype == OUT_F32;
-  switch (precision) {
-    case PREC_FP64:
-      return f32 ? time_e<double, float, PREC_FP64>(a, e, s) : time_e<double, double, PREC_FP64>(a, e, s);
-    case PREC_FP32:
-      return f32 ? time_e<float, float, PREC_FP32>(a, e, s) : time_e<float, double, PREC_FP32>(a, e, s);
-    case PREC_FP32C:
-      return f32 ? time_e<float, float, PREC_FP32C>(a, e, s) : time_e<float, double, PREC_FP32C>(a, e, s);
-    default: return hipErrorInvalidValue;
-  }
-}
 
 hipError_t launch_diff(const TimeArgs& a, int precision, int out_dtype, hipStream_t s) {
   const bool f32 = out_dtype == OUT_F32;

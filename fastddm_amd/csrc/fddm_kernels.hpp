@@ -55,12 +55,9 @@ struct TimeArgs {
 
 hipError_t launch_rows(const RowArgs& a, int precision, hipStream_t s);
 hipError_t launch_cols(const ColArgs& a, int precision, hipStream_t s);
-hipError_t launch_time(const TimeArgs& a, int precision, int out_dtype, hipStream_t s);
 hipError_t launch_diff(const TimeArgs& a, int precision, int out_dtype, hipStream_t s);
-// Time kernel at N = 2^logn (p2_plan): LDS bytes per workgroup, twiddle
-// table length (complex entries, see p2 twiddle layout) and launcher.
+// Time kernel at N = 2^logn (p2_plan): LDS bytes per workgroup and launcher.
 size_t p2_lds_bytes(int logn, int precision);
-int p2_twiddle_count(int logn);
 hipError_t launch_time_p2(const TimeArgs& a, int logn, int precision, int out_dtype, hipStream_t s);
 // Power-of-two spatial kernels (256 <= n <= 4096): plan radices and launchers.
 bool p2_spatial_ok(int logn);

@@ -3,23 +3,26 @@
 // D(q, dt) does not depend on the zero-padded transform length N as long as
 // N >= 2L - 1, so the time stage always runs at N = 2^ceil(log2(2L - 1))
 // (N >= 512) with plans fixed at compile time (p2_plan in fddm_common.hpp):
-// three passes per transform with radix 4..32 butterflies, all strides and
-// LDS offsets known to the compiler, and padded LDS addresses (one slot per
-// 32 elements) that stay linear in the butterfly index.
+// three or four passes per transform with radix 4..16 butterflies, all
+// strides and LDS offsets known to the compiler, and padded LDS addresses
+// (one slot per 32 elements) that stay linear in the butterfly index.  One
+// transform is worked on by 64 .. 1024 threads; plans of fewer than 256
+// threads put several q in one workgroup.
 //
 // Per q, one group of T threads:
 //   1. loads its first-pass inputs F(t), t = u + b T + r N/R0, straight from
 //      HBM (t >= N/2 >= L is the zero padding and is never read);
-//   2. compensated paths: subtracts the temporal mean, records the first and
-//      last kEdge samples, forms prefix sums of |F(t)|^2 in double (chunked
-//      wave scans);
-//   3. FFT1 (R0, R1, R2); the last pass stays in registers, |G|^2 is formed
-//      there and FFT2's first pass (radix R2, same distribution) runs on it;
-//   4. FFT2 (R2, R1, R0); the last pass leaves Re C(t) (times N) in the same
+//   2. compensated paths: subtracts the temporal mean (one group reduction);
+//   3. FFT1 (radices R0 .. RL); the last pass stays in registers, |G|^2 is
+//      formed there and FFT2's first pass (radix RL, same distribution) runs
+//      on it;
+//   4. FFT2 (RL .. R0); the last pass leaves Re C(t) (times N) in the same
 //      registers as the loaded F(t);
-//   5. writes C and the prefix sums to LDS, then evaluates D for every
-//      requested lag (direct sums when L - dt <= kEdge), PS and VAR, and
-//      stores them fftshifted along y.
+//   5. writes C(t) to LDS, re-reads the series (cache resident) for the
+//      prefix sums of |F(t)|^2 (double on the compensated paths; contiguous
+//      segments per thread and one group scan) and keeps the first and last
+//      kEdge samples; then evaluates D for every requested lag (direct sums
+//      when L - dt <= kEdge), PS and VAR, and stores them fftshifted along y.
 #include <type_traits>
 #include <utility>
 
@@ -30,8 +33,7 @@ namespace fddm {
 namespace {
 using namespace p2;
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
+constexpr int kMinThreads = 256;  // workgroup size for plans of fewer threads (G q per group)
 constexpr int kEdge = 32;
 
 template <typename A>
@@ -53,7 +55,7 @@ __device__ __forceinline__ A wave_incl_scan(A v) {
 }
 
 // Sums of x and y over the WPT waves of this thread's group (one barrier);
-// red holds 2 kWaves doubles and is written once per launch.
+// red holds two doubles per wave of the workgroup and is written once.
 template <int WPT>
 __device__ __forceinline__ void group_sum2(double& x, double& y, double* red) {
   const int wave = threadIdx.x >> 6;
@@ -75,50 +77,57 @@ __device__ __forceinline__ void group_sum2(double& x, double& y, double* red) {
   y = sy;
 }
 
-// Twiddle table offsets (complex entries): FFT1 passes 1, 2 then FFT2 passes 1, 2;
-// the pass with radix R after NS points holds exp(-2 pi i r k / (NS R)) at k R + r.
-template <int LOGN>
-struct P2Tw {
-  static constexpr P2Plan P = p2_plan(LOGN);
-  static constexpr int N = 1 << LOGN;
-  static constexpr int a1 = 0;
-  static constexpr int a2 = P.r0 * P.r1;
-  static constexpr int b1 = a2 + N;
-  static constexpr int b2 = b1 + P.r2 * P.r1;
-  static constexpr int total = b2 + N;
-};
-
+// Workgroup shape and LDS layout of the time kernel at N = 2^LOGN.
 template <int LOGN, typename T, bool SUB>
 struct P2Lds {
-  static constexpr P2Plan P = p2_plan(LOGN);
-  static constexpr int N = 1 << LOGN, TPT = P.threads, G = kThreads / TPT, WPT = TPT / 64;
-  static constexpr int MB0 = N / P.r0 / TPT;
-  static constexpr int NCH = (P.r0 / 2) * MB0;
+  static constexpr P2Plan P = p2_plan(LOGN, sizeof(T) == 8);
+  static constexpr int N = 1 << LOGN, TPT = P.threads;
+  static constexpr int NT = TPT > kMinThreads ? TPT : kMinThreads;  // workgroup size
+  static constexpr int G = NT / TPT, WPT = TPT / 64;
   using Acc = std::conditional_t<SUB, double, float>;
   static constexpr size_t csz = sizeof(cplx_t<T>);
   static constexpr size_t buf = static_cast<size_t>(G) * lds_padded_size(N) * csz;
   static constexpr size_t edge = static_cast<size_t>(G) * 2 * kEdge * csz;
-  static constexpr size_t red = 2 * kWaves * sizeof(double);
+  static constexpr size_t red = 2 * (NT / 64) * sizeof(double);
   static constexpr size_t chunk = static_cast<size_t>(G) * WPT * sizeof(Acc);
   static constexpr size_t bytes = buf + edge + red + chunk;
 };
 
+// Middle passes (LDS to LDS) of one transform: FFT1 (B = false) passes
+// I .. np-2, or FFT2 (B = true) passes I .. np-2.
+template <typename T, int LOGN, bool B, int I>
+__device__ __forceinline__ void p2_mid_passes(cplx_t<T>* buf, const cplx_t<T>* __restrict__ tw, int u) {
+  constexpr P2Plan P = p2_plan(LOGN, sizeof(T) == 8);
+  if constexpr (I < P.np - 1) {
+    constexpr int N = 1 << LOGN;
+    constexpr int R = B ? p2_radix_b(P, I) : p2_radix(P, I);
+    constexpr int NS = B ? p2_ns_b(P, I) : p2_ns(P, I);
+    constexpr int OFF = B ? p2_tw_b(P, I) : p2_tw_a(P, I);
+    using Ps = Pass<T, N, R, NS, P.threads>;
+    cplx_t<T> v[Ps::MB][R];
+    Ps::compute(buf, v, tw + OFF, u);
+    __syncthreads();
+    Ps::store(buf, v, u);
+    __syncthreads();
+    p2_mid_passes<T, LOGN, B, I + 1>(buf, tw, u);
+  }
+}
+
 template <typename T, typename OutT, int LOGN, bool SUB>
-__global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd, int nq) {
+__global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs a, int wg_per_xcd, int nq) {
   using C = cplx_t<T>;
   using Lay = P2Lds<LOGN, T, SUB>;
   using Acc = typename Lay::Acc;
-  using Tw = P2Tw<LOGN>;
-  constexpr P2Plan P = p2_plan(LOGN);
-  constexpr int N = 1 << LOGN, TPT = P.threads, G = Lay::G, WPT = Lay::WPT;
-  constexpr int R0 = P.r0, R1 = P.r1, R2 = P.r2;
-  using A0 = Pass<T, N, R0, 1, TPT>;
-  using A1 = Pass<T, N, R1, R0, TPT>;
-  using A2 = Pass<T, N, R2, R0 * R1, TPT>;
-  using B0 = Pass<T, N, R2, 1, TPT>;
-  using B1 = Pass<T, N, R1, R2, TPT>;
-  using B2 = Pass<T, N, R0, R2 * R1, TPT>;
-  static_assert(A2::MB == B0::MB && B2::MB == A0::MB, "register hand-offs");
+  constexpr P2Plan P = Lay::P;
+  constexpr int N = 1 << LOGN, TPT = P.threads, G = Lay::G, WPT = Lay::WPT, NP = P.np;
+  constexpr int R0 = P.r0, RL = p2_radix(P, NP - 1);
+  using A0 = Pass<T, N, R0, 1, TPT>;                          // FFT1 first pass (registers)
+  using AL = Pass<T, N, RL, N / RL, TPT>;                     // FFT1 last pass (to registers)
+  using B0 = Pass<T, N, RL, 1, TPT>;                          // FFT2 first pass (registers)
+  using BL = Pass<T, N, R0, N / R0, TPT>;                     // FFT2 last pass (to registers)
+  static_assert(AL::MB == B0::MB && BL::MB == A0::MB, "register hand-offs");
+  static_assert(N % (R0 * TPT) == 0, "whole first-pass butterflies per thread");
+  constexpr int TWA = p2_tw_a(P, NP - 1), TWB = p2_tw_b(P, NP - 1);
   constexpr int MB0 = A0::MB, M0 = A0::M, RH = R0 / 2;
 
   extern __shared__ __align__(16) unsigned char smem[];
@@ -175,31 +184,24 @@ __global__ void __launch_bounds__(kThreads) k_time_p2(TimeArgs a, int wg_per_xcd
   const double fsr = sr - static_cast<double>(L) * static_cast<double>(m_r);
   const double fsi = si - static_cast<double>(L) * static_cast<double>(m_i);
 
-  // 3. FFT1; its last pass stays in registers
+  // 3. FFT1: first pass on the registers, last pass back into registers
   A0::compute_regs(v0);
   A0::store(buf, v0, u);
   __syncthreads();
-  C v1[A1::MB][R1];
-  A1::compute(buf, v1, tw + Tw::a1, u);
-  __syncthreads();
-  A1::store(buf, v1, u);
-  __syncthreads();
-  C v2[A2::MB][R2];
-  A2::compute(buf, v2, tw + Tw::a2, u);
+  p2_mid_passes<T, LOGN, false, 1>(buf, tw, u);
+  C v2[AL::MB][RL];
+  AL::compute(buf, v2, tw + TWA, u);
 #pragma unroll
-  for (int b = 0; b < A2::MB; ++b)
+  for (int b = 0; b < AL::MB; ++b)
 #pragma unroll
-    for (int r = 0; r < R2; ++r) v2[b][r] = C{v2[b][r].x * v2[b][r].x + v2[b][r].y * v2[b][r].y, 0};
-  // 4. FFT2; first pass on the registers, last pass back into registers
+    for (int r = 0; r < RL; ++r) v2[b][r] = C{v2[b][r].x * v2[b][r].x + v2[b][r].y * v2[b][r].y, 0};
+  // 4. FFT2 of |G|^2, same register hand-offs
   B0::compute_regs(v2);
   __syncthreads();
   B0::store(buf, v2, u);
   __syncthreads();
-  B1::compute(buf, v1, tw + Tw::b1, u);
-  __syncthreads();
-  B1::store(buf, v1, u);
-  __syncthreads();
-  B2::compute(buf, v0, tw + Tw::b2, u);
+  p2_mid_passes<T, LOGN, true, 1>(buf, tw, u);
+  BL::compute(buf, v0, tw + TWB, u);
   __syncthreads();
 
   // 5. C(t) N into LDS; prefix sums psum(t + 1) of |F(t) - mean|^2 from a
@@ -320,7 +322,7 @@ hipError_t launch_one(const TimeArgs& a, hipStream_t s) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(kernel, dim3(8u * wpx), dim3(kThreads), lds, s, a, wpx, nq);
+  hipLaunchKernelGGL(kernel, dim3(8u * wpx), dim3(Lay::NT), lds, s, a, wpx, nq);
   return hipGetLastError();
 }
 
@@ -358,18 +360,6 @@ size_t p2_lds_bytes(int logn, int precision) {
   if (precision == PREC_FP64) return pick(double{}, std::true_type{});
   if (precision == PREC_FP32) return pick(float{}, std::false_type{});
   return pick(float{}, std::true_type{});
-}
-
-int p2_twiddle_count(int logn) {
-  switch (logn) {
-    case 9: return P2Tw<9>::total;
-    case 10: return P2Tw<10>::total;
-    case 11: return P2Tw<11>::total;
-    case 12: return P2Tw<12>::total;
-    case 13: return P2Tw<13>::total;
-    case 14: return P2Tw<14>::total;
-    default: return 0;
-  }
 }
 
 hipError_t launch_time_p2(const TimeArgs& a, int logn, int precision, int out_dtype, hipStream_t s) {

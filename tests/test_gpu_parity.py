@@ -129,6 +129,39 @@ def test_whole_path_matches_oracle(hip, mode, case, precision):
     CHECK[precision](out, ref)
 
 
+# Every power-of-two time plan (N = 512 .. 16384: three- and four-pass plans,
+# 64 .. 1024 threads per transform) on a small image.
+TIME_LENGTHS = [(200, 512), (300, 1024), (700, 2048), (1500, 4096), (3000, 8192), (5000, 16384)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c", "fp32"])
+@pytest.mark.parametrize("L,N", TIME_LENGTHS)
+def test_every_time_plan_matches_oracle(hip, L, N, precision):
+    assert 2 * L - 1 <= N < 4 * L - 2 or N == 512
+    img = ar1_speckle(L, 12, 10, sigma=1.2, diffusivity=0.05, mean=800, std=120, seed=L)
+    lags = np.unique(np.concatenate([np.arange(1, 40), np.geomspace(40, L - 1, 60).astype(int),
+                                     np.arange(L - 40, L)]))
+    nt = next_fast_len(2 * L, "hip")
+    if precision == "fp64" and L > 4096:
+        with pytest.raises(RuntimeError, match="too long"):
+            hip.ddm_fft(img, lags, 10, 12, nt, None, precision=precision)
+        return
+    ref = onp.ddm_fft(img, lags, 10, 12, 2 * L, None, subtract_mean=True)
+    out = hip.ddm_fft(img, lags, 10, 12, nt, None, precision=precision)
+    CHECK[precision](out, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+@pytest.mark.parametrize("L", [1500, 5000])
+def test_diff_mode_long_series(hip, L, precision):
+    """Series longer than the LDS-staged diff kernel takes (fp64 above 4096
+    frames, both precisions at 5000 on the direct-read path)."""
+    img = ar1_speckle(L, 8, 9, sigma=1.0, diffusivity=0.1, mean=600, std=90, seed=L + 1)
+    lags = np.unique(np.geomspace(1, L - 1, 50).astype(int))
+    ref = onp.ddm_fft(img, lags, 9, 8, 2 * L, None, subtract_mean=True)
+    CHECK[precision](hip.ddm_diff(img, lags, 9, 8, None, precision=precision), ref)
+
+
 @pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64,
                                    np.uint64, np.float32, np.float64])
 def test_every_input_dtype(hip, dtype):
