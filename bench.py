@@ -106,6 +106,114 @@ def cpu_baseline(img, lags, nx, ny, nt, win, full_L, nq):
     }
 
 
+def serve_main(args):
+    """Serving run: Poisson arrivals of mixed analyses into the request server.
+
+    500 requests at a fixed rate; size uniform in {128, 256, 512, 1024}^2,
+    frames log-uniform in [100, 4000], 80 % fft (all lags) / 20 % diff (50
+    log-spaced lags), fp32c, uint16, float64 output kept on device until the
+    client releases it; five requests are forced to 1024^2 x 4000 fft.  Inputs
+    are device-resident stacks of 4000 frames per size (uniform integers in
+    [800, 1200) made on the device); request k uses the first L_k frames.
+    Latency = completion seen by the server - submission (plan creation
+    included).
+    """
+    import threading
+
+    from fastddm_amd import _core_hip as core
+    from fastddm_amd import next_fast_len
+    from fastddm_amd.lags import logspace_int
+
+    core.set_device(0)
+    rng = np.random.default_rng(5)
+    n, rate = args.serve_requests, args.serve_rate
+    sizes = np.array([128, 256, 512, 1024])
+    size = sizes[rng.integers(0, 4, n)]
+    L = np.exp(rng.uniform(np.log(100), np.log(4000), n)).round().astype(int)
+    mode = np.where(rng.random(n) < 0.8, "fft", "diff")
+    forced = np.linspace(0, n - 1, 7).astype(int)[1:-1]  # five evenly spread
+    size[forced], L[forced], mode[forced] = 1024, 4000, "fft"
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+
+    stacks = {}
+    for k, sz in enumerate(sizes):
+        b = core.DeviceBuffer(4000 * int(sz) * int(sz) * 2)
+        core.fill_uniform(b, 4000 * int(sz) * int(sz), np.uint16, 1000 + k, 800, 1200)
+        stacks[int(sz)] = b
+    srv = core.Server(max_slots=args.serve_slots)
+    rid_of = {}
+    done = {}
+    lock = threading.Lock()
+
+    def drain():
+        while True:
+            with lock:
+                if len(done) == n:
+                    return
+            rid = srv.next_done(0.5)
+            if rid == 0:
+                continue
+            st, t = srv.status(rid)
+            with lock:
+                done[rid] = (st, t)
+            srv.release(rid)
+
+    th = threading.Thread(target=drain, daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    for i in range(n):
+        wait = arrivals[i] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        sz, l = int(size[i]), int(L[i])
+        lags = np.arange(1, l) if mode[i] == "fft" else logspace_int(l - 1, num=50, endpoint=True)
+        nt = next_fast_len(2 * l, "hip") if mode[i] == "fft" else 0
+        rid = srv.submit(mode[i], stacks[sz], lags, sz, sz, nt, precision="fp32c", img_on_device=True,
+                         shape=(l, sz, sz), dtype=np.uint16)
+        rid_of[i] = rid
+    th.join(timeout=600)
+    stats = srv.stats()
+    srv.close()
+    for b in stacks.values():
+        b.free()
+    if len(done) != n:
+        raise RuntimeError(f"only {len(done)} of {n} requests finished")
+    lat = np.empty(n)
+    first, last = np.inf, 0.0
+    for i in range(n):
+        st, (ts, ta, td) = done[rid_of[i]]
+        if st != core.Server.DONE:
+            raise RuntimeError(f"request {i} failed")
+        lat[i] = (td - ts) * 1e3
+        first, last = min(first, ts), max(last, td)
+    frames = int(L.sum())
+    per_size = {}
+    for sz in sizes:
+        m = size == sz
+        per_size[f"{sz}x{sz}"] = {"requests": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50)),
+                                 "p99_ms": float(np.percentile(lat[m], 99))}
+    return {
+        "requests": n,
+        "arrival_rate_per_s": rate,
+        "slots": args.serve_slots,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(lat.mean()),
+        "max_ms": float(lat.max()),
+        "sustained_frames_per_s": frames / (last - first),
+        "frames": frames,
+        "span_s": last - first,
+        "fft_fraction": float(np.mean(mode == "fft")),
+        "forced_1024x4000": int(len(forced)),
+        "arena_GB": stats["arena_bytes"] / 1e9,
+        "arena_peak_GB": stats["arena_peak"] / 1e9,
+        "plans": stats["plans"],
+        "per_size": per_size,
+        "note": ("device-resident uint16 stacks (uniform integers), fp32c, float64 results kept on "
+                 "device until released; latency includes plan creation"),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +226,11 @@ def main():
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=500)
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device-to-device copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -126,6 +239,26 @@ def main():
               "(whole stacks would be routed to GPUs independently). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.serve_only:
+        print(json.dumps(serve_main(args)), flush=True)
+        return
+    serve = None
+    if not args.skip_serve and not args.copy_calibration:
+        # in a child process started before this one touches the GPU
+        import subprocess
+
+        cmd = [sys.executable, os.path.abspath(__file__), "--serve-only",
+               "--serve-requests", str(args.serve_requests), "--serve-rate", str(args.serve_rate),
+               "--serve-slots", str(args.serve_slots)]
+        t0 = time.perf_counter()
+        try:
+            res = subprocess.run(cmd, capture_output=True, text=True, timeout=420)
+            lines = [l for l in res.stdout.splitlines() if l.startswith("{")]
+            serve = json.loads(lines[-1]) if res.returncode == 0 and lines else {
+                "error": f"serving run exited with {res.returncode}: {res.stderr[-400:]}"}
+        except subprocess.TimeoutExpired:
+            serve = {"error": "serving run exceeded 420 s"}
+        log(f"serving run: {time.perf_counter() - t0:.1f} s")
 
     from fastddm_amd import _core_hip as core
     from fastddm_amd import next_fast_len
@@ -259,6 +392,8 @@ def main():
             "frac_of_hbm_peak": b["whole"] / (ms32 / 1e3) / 1e9 / HBM_PEAK_GBS,
         },
     }
+    if serve is not None:
+        result["serve"] = serve
     if fp64 is not None:
         result["fp64"] = fp64
         result["speedup_vs_fp64"] = fp64["ms_per_step"] / ms32

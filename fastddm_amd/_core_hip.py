@@ -73,6 +73,19 @@ def lib() -> ctypes.CDLL:
         "fddm_stream_destroy": [p],
         "fddm_stream_synchronize": [p, cp, sz],
         "fddm_test_fft": [p, u64, u64, i32, cp, sz],
+        "fddm_server_create": [ctypes.POINTER(p), i32, u64, cp, sz],
+        "fddm_server_destroy": [p],
+        "fddm_server_set_aging": [p, ctypes.c_double],
+        "fddm_server_submit": [p, i32, p, i32, i32, u64, u64, u64, p, u64, u64, u64, u64, p, i32,
+                               i32, i32, ctypes.POINTER(u64), cp, sz],
+        "fddm_server_status": [p, u64, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double), cp, sz],
+        "fddm_server_wait": [p, u64, ctypes.c_double, cp, sz],
+        "fddm_server_next_done": [p, ctypes.c_double, ctypes.POINTER(u64), cp, sz],
+        "fddm_server_result": [p, u64, ctypes.POINTER(p), ctypes.POINTER(u64), cp, sz],
+        "fddm_server_fetch": [p, u64, p, cp, sz],
+        "fddm_server_release": [p, u64],
+        "fddm_server_stats": [p, ctypes.POINTER(u64)],
+        "fddm_fill_uniform": [p, u64, i32, u64, ctypes.c_uint32, ctypes.c_uint32, p, cp, sz],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -107,6 +120,18 @@ EXPORTED_SYMBOLS = (
     "fddm_stream_destroy",
     "fddm_stream_synchronize",
     "fddm_test_fft",
+    "fddm_server_create",
+    "fddm_server_destroy",
+    "fddm_server_set_aging",
+    "fddm_server_submit",
+    "fddm_server_status",
+    "fddm_server_wait",
+    "fddm_server_next_done",
+    "fddm_server_result",
+    "fddm_server_fetch",
+    "fddm_server_release",
+    "fddm_server_stats",
+    "fddm_fill_uniform",
 )
 
 
@@ -344,5 +369,99 @@ class Plan:
     def __del__(self):
         try:
             self.destroy()
+        except Exception:
+            pass
+
+
+def fill_uniform(buf: "DeviceBuffer", count: int, dtype, seed: int, lo: int, hi: int) -> None:
+    """Fill ``count`` uint8/uint16 elements of a device buffer with uniform
+    integers in [lo, hi) (device-side load generation)."""
+    if count * np.dtype(dtype).itemsize > buf.nbytes:
+        raise RuntimeError("fill_uniform: buffer too small")
+    _call("fddm_fill_uniform", buf.ptr, count, DTYPE_CODES[np.dtype(dtype)], seed, lo, hi, None)
+    _call("fddm_stream_synchronize", None)
+
+
+class Server:
+    """Continuous-batching request server on the current device.
+
+    ``submit`` returns a request id at once; the request is admitted when a
+    slot is free and its HBM footprint fits in the server's arena, and runs
+    concurrently with the other admitted requests.  ``fetch`` copies a
+    finished result to host, ``release`` frees it.
+    """
+
+    PENDING, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, max_slots: int = 4, hbm_bytes: int = 0, aging_s: Optional[float] = None):
+        h = ctypes.c_void_p()
+        _call("fddm_server_create", ctypes.byref(h), max_slots, hbm_bytes)
+        self._h = h
+        self._keep = {}
+        if aging_s is not None:
+            lib().fddm_server_set_aging(self._h, aging_s)
+
+    def submit(self, mode, img, lags, nx, ny, nt=0, window=None, precision="fp32c",
+               out_dtype=np.float64, img_on_device=False, shape=None, dtype=None) -> int:
+        """Queue one analysis.  ``img`` is a host array (read at admission; kept
+        alive here until release) or, with ``img_on_device``, a DeviceBuffer
+        holding a (L, H, W) stack of ``dtype`` given by ``shape``."""
+        lg = _lags_arg(lags)
+        wp, wf32, wkeep = _window_arg(window)
+        if img_on_device:
+            L, H, W = shape
+            ptr, code = img.ptr, DTYPE_CODES[np.dtype(dtype)]
+        else:
+            img = _img_arg(img)
+            L, H, W = img.shape
+            ptr, code = img.ctypes.data, DTYPE_CODES[img.dtype]
+        rid = ctypes.c_uint64()
+        _call("fddm_server_submit", self._h, MODE_FFT if mode == "fft" else MODE_DIFF, ptr,
+              1 if img_on_device else 0, code, L, H, W, lg.ctypes.data, len(lg), nx, ny, nt, wp,
+              wf32, _prec_code(precision), _out_code(out_dtype), ctypes.byref(rid))
+        self._keep[rid.value] = (img, (len(lg) + 2, ny, nx // 2 + 1), np.dtype(out_dtype))
+        return rid.value
+
+    def status(self, rid: int):
+        st = ctypes.c_int()
+        t = (ctypes.c_double * 3)()
+        _call("fddm_server_status", self._h, rid, ctypes.byref(st), t)
+        return st.value, (t[0], t[1], t[2])
+
+    def wait(self, rid: int, timeout_s: float = -1.0) -> None:
+        _call("fddm_server_wait", self._h, rid, timeout_s)
+
+    def next_done(self, timeout_s: float = -1.0) -> int:
+        """Id of the next finished request (completion order); 0 on timeout."""
+        rid = ctypes.c_uint64()
+        _call("fddm_server_next_done", self._h, timeout_s, ctypes.byref(rid))
+        return rid.value
+
+    def fetch(self, rid: int) -> np.ndarray:
+        _, shape, dt = self._keep[rid]
+        out = np.empty(shape, dtype=dt)
+        _call("fddm_server_fetch", self._h, rid, out.ctypes.data)
+        return out
+
+    def release(self, rid: int) -> None:
+        if lib().fddm_server_release(self._h, rid) != 0:
+            raise RuntimeError(f"request {rid} cannot be released (unknown or still running)")
+        self._keep.pop(rid, None)
+
+    def stats(self) -> dict:
+        v = (ctypes.c_uint64 * 7)()
+        lib().fddm_server_stats(self._h, v)
+        keys = ("pending", "running", "completed", "arena_bytes", "arena_used", "arena_peak", "plans")
+        return dict(zip(keys, list(v)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().fddm_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

@@ -128,6 +128,51 @@ int fddm_stream_create(void **stream, char *err, size_t errlen);
 int fddm_stream_destroy(void *stream);
 int fddm_stream_synchronize(void *stream, char *err, size_t errlen);
 
+/* ---- request server (continuous batching on one GPU) ----
+ * No counterpart in the reference, whose cores run one analysis per blocking
+ * call; this is the serving layer around fddm_plan_*.  A server owns one
+ * device arena of hbm_bytes (0: 85 % of free memory) and max_slots worker
+ * streams on the device current at creation.  A submitted request is
+ * admitted as soon as a slot is free and its footprint (plan workspace +
+ * output [+ a device copy of a host input]) fits in the arena; admitted
+ * requests run concurrently.  Results stay in the arena until released.
+ * img_on_device: img is a device pointer that stays valid until the request
+ * finishes; otherwise a host pointer read when the request is admitted.
+ * States: 0 pending, 1 running, 2 done, 3 failed.  times[3] = seconds since
+ * server start at submit, admission and completion. */
+typedef struct fddm_server fddm_server;
+int fddm_server_create(fddm_server **server, int max_slots, uint64_t hbm_bytes, char *err,
+                       size_t errlen);
+/* Drops requests not yet admitted, waits for running ones, frees everything. */
+int fddm_server_destroy(fddm_server *server);
+/* A pending request older than `seconds` blocks later ones from passing it. */
+int fddm_server_set_aging(fddm_server *server, double seconds);
+int fddm_server_submit(fddm_server *server, int mode, const void *img, int img_on_device, int dtype,
+                       uint64_t length, uint64_t height, uint64_t width, const uint32_t *lags,
+                       uint64_t num_lags, uint64_t nx, uint64_t ny, uint64_t nt, const void *window,
+                       int window_f32, int precision, int out_dtype, uint64_t *id, char *err,
+                       size_t errlen);
+int fddm_server_status(fddm_server *server, uint64_t id, int *state, double *times, char *err,
+                       size_t errlen);
+/* timeout_s < 0 waits without limit. */
+int fddm_server_wait(fddm_server *server, uint64_t id, double timeout_s, char *err, size_t errlen);
+/* Id of the next request that finished (done or failed), in completion
+ * order, each reported once; *id = 0 when timeout_s (>= 0) passes first. */
+int fddm_server_next_done(fddm_server *server, double timeout_s, uint64_t *id, char *err,
+                          size_t errlen);
+/* Device pointer and size of a finished result (valid until release). */
+int fddm_server_result(fddm_server *server, uint64_t id, void **d_out, uint64_t *bytes, char *err,
+                       size_t errlen);
+int fddm_server_fetch(fddm_server *server, uint64_t id, void *host_out, char *err, size_t errlen);
+int fddm_server_release(fddm_server *server, uint64_t id);
+/* stats[7] = {pending, running, completed, arena bytes, arena used, arena
+ * peak, cached plans}. */
+int fddm_server_stats(fddm_server *server, uint64_t *stats);
+/* Load generation: fills count uint8/uint16 elements of a device buffer with
+ * uniform integers in [lo, hi) from a counter-based hash of (seed, index). */
+int fddm_fill_uniform(void *d, uint64_t count, int dtype, uint64_t seed, uint32_t lo, uint32_t hi,
+                      void *stream, char *err, size_t errlen);
+
 /* ---- FFT engine test entry: batch forward FFTs of length n, in place on
  * host data (interleaved double complex); precision FP64 or FP32. */
 int fddm_test_fft(double *data, uint64_t n, uint64_t batch, int precision, char *err,
