@@ -106,6 +106,80 @@ def cpu_baseline(img, lags, nx, ny, nt, win, full_L, nq):
     }
 
 
+def batch_main(args):
+    """Uneven stacks in one launch: 48 stacks, (H, W) cycling through 96x96,
+    150x200, 384x384, 480x640, frames log-uniform in [50, 1500] (seed 4),
+    fft mode, all lags, fp32c, float64 output; uint16 inputs made on the
+    device (uniform integers in [800, 1200)).  One step = one replay of the
+    batch graph; the same plans run one after another without the graph are
+    timed beside it."""
+    from fastddm_amd import _core_hip as core
+    from fastddm_amd import next_fast_len
+
+    core.set_device(0)
+    rng = np.random.default_rng(4)
+    shapes = [(96, 96), (150, 200), (384, 384), (480, 640)]
+    n = 48
+    Ls = np.exp(rng.uniform(np.log(50), np.log(1500), n)).round().astype(int)
+    items = []
+    t0 = time.perf_counter()
+    for k in range(n):
+        H, W = shapes[k % 4]
+        L = int(Ls[k])
+        img = core.DeviceBuffer(L * H * W * 2)
+        core.fill_uniform(img, L * H * W, np.uint16, 7000 + k, 800, 1200)
+        lags = np.arange(1, L)
+        nt = next_fast_len(2 * L, "hip")
+        plan = core.Plan("fft", np.uint16, (L, H, W), lags, W, H, nt, None, "fp32c")
+        out = core.DeviceBuffer(plan.output_size())
+        plan.destroy()
+        items.append(dict(mode="fft", img=img, shape=(L, H, W), dtype=np.uint16, lags=lags, nx=W, ny=H,
+                          nt=nt, precision="fp32c", out=out))
+    batch = core.Batch(items, autotune=True)
+    log(f"batch of {n} stacks built (plans, autotune, graph) in {time.perf_counter() - t0:.1f} s")
+    stream = core.Stream()
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn(stream)
+        stream.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            fn(stream)
+        stream.synchronize()
+        return (time.perf_counter() - t) / args.steps * 1e3
+
+    ms_graph = timed(batch.launch)
+    ms_eager = timed(batch.launch_eager)
+    tbs = batch.frames_per_block()
+    frames = int(Ls.sum())
+    batch.close()
+    stream.destroy()
+    for it in items:
+        it["img"].free()
+        it["out"].free()
+    return {
+        "metric": "batched fft-mode frames/s, 48 uneven stacks in one graph launch",
+        "value": frames / (ms_graph / 1e3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_graph,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "device-made uniform uint16 stacks (seeded hash), shapes and lengths seeded",
+        "config": {"workload": ("48 stacks, (H, W) cycling 96x96 / 150x200 / 384x384 / 480x640, "
+                                "frames log-uniform in [50, 1500], fft mode, all lags, fp32c, float64 out"),
+                   "frames": frames, "stacks": n, "parallelism": "single GPU"},
+        "eager_ms_per_step": ms_eager,
+        "graph_speedup_vs_eager": ms_eager / ms_graph,
+        "frames_per_block_by_shape": {f"{h}x{w}": tbs[k] for k, (h, w) in enumerate(shapes)},
+    }
+
+
 def serve_main(args):
     """Serving run: Poisson arrivals of mixed analyses into the request server.
 
@@ -219,7 +293,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["fft", "diff"], default="fft",
+    ap.add_argument("--workload", choices=["fft", "diff", "batch"], default="fft",
                     help="fft: 2000 x 512^2 uint16, all lags (default); diff: 300 x 256^2 uint8, "
                          "50 log-spaced lags, no window")
     ap.add_argument("--frames", type=int, default=None)
@@ -242,8 +316,11 @@ def main():
     if args.serve_only:
         print(json.dumps(serve_main(args)), flush=True)
         return
+    if args.workload == "batch":
+        print(json.dumps(batch_main(args)), flush=True)
+        return
     serve = None
-    if not args.skip_serve and not args.copy_calibration:
+    if not args.skip_serve and not args.copy_calibration and args.workload == "fft":
         # in a child process started before this one touches the GPU
         import subprocess
 

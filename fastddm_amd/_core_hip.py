@@ -86,6 +86,11 @@ def lib() -> ctypes.CDLL:
         "fddm_server_release": [p, u64],
         "fddm_server_stats": [p, ctypes.POINTER(u64)],
         "fddm_fill_uniform": [p, u64, i32, u64, ctypes.c_uint32, ctypes.c_uint32, p, cp, sz],
+        "fddm_batch_create": [ctypes.POINTER(p), p, i32, i32, cp, sz],
+        "fddm_batch_launch": [p, p, cp, sz],
+        "fddm_batch_launch_eager": [p, p, cp, sz],
+        "fddm_batch_frames_per_block": [p, ctypes.POINTER(i32)],
+        "fddm_batch_destroy": [p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -132,6 +137,11 @@ EXPORTED_SYMBOLS = (
     "fddm_server_release",
     "fddm_server_stats",
     "fddm_fill_uniform",
+    "fddm_batch_create",
+    "fddm_batch_launch",
+    "fddm_batch_launch_eager",
+    "fddm_batch_frames_per_block",
+    "fddm_batch_destroy",
 )
 
 
@@ -459,6 +469,70 @@ class Server:
             lib().fddm_server_destroy(self._h)
             self._h = None
             self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _BatchItem(ctypes.Structure):
+    """Mirror of ``fddm_batch_item`` (include/fddm_hip.h)."""
+
+    _fields_ = [
+        ("mode", ctypes.c_int), ("dtype", ctypes.c_int),
+        ("length", ctypes.c_uint64), ("height", ctypes.c_uint64), ("width", ctypes.c_uint64),
+        ("lags", ctypes.c_void_p), ("num_lags", ctypes.c_uint64),
+        ("nx", ctypes.c_uint64), ("ny", ctypes.c_uint64), ("nt", ctypes.c_uint64),
+        ("window", ctypes.c_void_p), ("window_f32", ctypes.c_int),
+        ("precision", ctypes.c_int), ("out_dtype", ctypes.c_int),
+        ("d_img", ctypes.c_void_p), ("d_out", ctypes.c_void_p),
+    ]
+
+
+class Batch:
+    """Many device-resident stacks analysed by one graph launch.
+
+    ``items``: dicts with keys mode, img (DeviceBuffer), shape (L, H, W),
+    dtype, lags, nx, ny, nt (fft), window (optional), precision, out_dtype
+    (optional) and out (DeviceBuffer of the plan's output size).
+    """
+
+    def __init__(self, items, autotune: bool = True):
+        n = len(items)
+        arr = (_BatchItem * n)()
+        self._keep = []
+        for k, it in enumerate(items):
+            lg = _lags_arg(it["lags"])
+            wp, wf32, wkeep = _window_arg(it.get("window"))
+            self._keep.append((lg, wkeep, it["img"], it["out"]))
+            L, H, W = it["shape"]
+            arr[k] = _BatchItem(
+                MODE_FFT if it["mode"] == "fft" else MODE_DIFF, DTYPE_CODES[np.dtype(it["dtype"])],
+                L, H, W, lg.ctypes.data, len(lg), it["nx"], it["ny"], it.get("nt", 0), wp, wf32,
+                _prec_code(it.get("precision", "fp32c")), _out_code(it.get("out_dtype", np.float64)),
+                it["img"].ptr, it["out"].ptr)
+        self._n = n
+        self._h = ctypes.c_void_p()
+        _call("fddm_batch_create", ctypes.byref(self._h), ctypes.cast(arr, ctypes.c_void_p), n,
+              1 if autotune else 0)
+
+    def launch(self, stream: Optional["Stream"] = None) -> None:
+        _call("fddm_batch_launch", self._h, None if stream is None else stream.handle)
+
+    def launch_eager(self, stream: Optional["Stream"] = None) -> None:
+        _call("fddm_batch_launch_eager", self._h, None if stream is None else stream.handle)
+
+    def frames_per_block(self):
+        v = (ctypes.c_int * self._n)()
+        lib().fddm_batch_frames_per_block(self._h, v)
+        return list(v)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().fddm_batch_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:

@@ -17,6 +17,7 @@
 
 #include "fddm_hip.h"
 #include "fddm_kernels.hpp"
+#include "fddm_plan.hpp"
 
 using namespace fddm;
 
@@ -218,10 +219,16 @@ void plan_free_device(fddm_plan* p) {
   p->events.clear();
 }
 
-int plan_create(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H, uint64_t W,
-                const uint32_t* lags, uint64_t nl, uint64_t nx, uint64_t ny, uint64_t nt,
-                const void* window, int window_f32, int precision, int out_dtype, char* err,
-                size_t errlen) {
+}  // namespace
+
+int fddm::plan_frames_per_block(const fddm_plan* p) { return p ? p->Tb : 0; }
+
+// frames_per_block: 0 picks the default block size; 2, 4, 8 or 16 forces it
+// (the autotuner of fddm_batch.hip tries several).
+int fddm::plan_create_ex(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H, uint64_t W,
+                         const uint32_t* lags, uint64_t nl, uint64_t nx, uint64_t ny, uint64_t nt,
+                         const void* window, int window_f32, int precision, int out_dtype, int frames_per_block,
+                         char* err, size_t errlen) {
   if (!out) return set_err(err, errlen, FDDM_ERR_ARG, "plan pointer is NULL");
   *out = nullptr;
   if (mode != MODE_FFT && mode != MODE_DIFF) return set_err(err, errlen, FDDM_ERR_ARG, "unknown mode %d", mode);
@@ -273,8 +280,14 @@ int plan_create(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H, ui
   // Frames per block: runs of Tb * sizeof(complex) = 128 B in the q-major
   // store, halved while a column slab (ny x Tb complex) exceeds 80 KiB (two
   // slabs per CU).
-  p->Tb = f64 ? 8 : 16;
-  while (p->Tb > 2 && lds_padded_size(static_cast<int>(ny) * p->Tb) * csz > 80 * 1024) p->Tb /= 2;
+  if (frames_per_block) {
+    if (frames_per_block != 2 && frames_per_block != 4 && frames_per_block != 8 && frames_per_block != 16)
+      return fail(set_err(err, errlen, FDDM_ERR_ARG, "frames per block must be 2, 4, 8 or 16"));
+    p->Tb = frames_per_block;
+  } else {
+    p->Tb = f64 ? 8 : 16;
+    while (p->Tb > 2 && lds_padded_size(static_cast<int>(ny) * p->Tb) * csz > 80 * 1024) p->Tb /= 2;
+  }
   if (lds_padded_size(static_cast<int>(ny) * p->Tb) * csz > kLdsLimit)
     return fail(set_err(err, errlen, FDDM_ERR_SIZE, "ny=%llu too large for the column pass", (unsigned long long)ny));
   // Transforms handled at once by the row / column kernels (<= 16 or 32
@@ -364,6 +377,8 @@ int plan_create(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H, ui
   *out = p;
   return FDDM_OK;
 }
+
+namespace {
 
 uint64_t output_bytes(const fddm_plan* p) {
   return (p->nl + 2) * p->nq * (p->out_dtype == OUT_F32 ? 4 : 8);
@@ -492,8 +507,8 @@ int host_path(int mode, const void* img, int dtype, uint64_t L, uint64_t H, uint
   FDDM_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) return set_err(err, errlen, FDDM_ERR_DEVICE, "device id %d not supported", dev);
   fddm_plan* p = nullptr;
-  int rc = plan_create(&p, mode, dtype, L, H, W, lags, nl, nx, ny, nt, window, window_f32, precision,
-                       out_dtype, err, errlen);
+  int rc = plan_create_ex(&p, mode, dtype, L, H, W, lags, nl, nx, ny, nt, window, window_f32, precision,
+                          out_dtype, 0, err, errlen);
   if (rc) return rc;
   struct PlanGuard {
     fddm_plan* p;
@@ -598,8 +613,8 @@ int fddm_plan_create(fddm_plan** plan, int mode, int dtype, uint64_t length, uin
                      const uint32_t* lags, uint64_t num_lags, uint64_t nx, uint64_t ny, uint64_t nt,
                      const void* window, int window_f32, int precision, int out_dtype, char* err,
                      size_t errlen) {
-  return plan_create(plan, mode, dtype, length, height, width, lags, num_lags, nx, ny, nt, window, window_f32,
-                     precision, out_dtype, err, errlen);
+  return plan_create_ex(plan, mode, dtype, length, height, width, lags, num_lags, nx, ny, nt, window, window_f32,
+                        precision, out_dtype, 0, err, errlen);
 }
 
 int fddm_plan_destroy(fddm_plan* plan) {

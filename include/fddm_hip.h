@@ -173,6 +173,38 @@ int fddm_server_stats(fddm_server *server, uint64_t *stats);
 int fddm_fill_uniform(void *d, uint64_t count, int dtype, uint64_t seed, uint32_t lo, uint32_t hi,
                       void *stream, char *err, size_t errlen);
 
+/* ---- batches: many stacks of uneven size in one graph launch ----
+ * No counterpart in the reference (one stack per call).  Each item is one
+ * analysis with device-resident input and output (output of
+ * fddm_plan_output_size bytes for the same arguments).  Creation builds one
+ * plan per item -- with autotune != 0 the frames-per-block of each image
+ * shape is chosen by timing the candidates on the item's first frames
+ * (cached per shape for the process) -- allocates one workspace for all
+ * items and captures every item's stage chain into one hipGraph on parallel
+ * branches.  fddm_batch_launch replays the graph on `stream`;
+ * fddm_batch_launch_eager runs the same plans one after another without the
+ * graph.  Inputs are read and outputs written at each launch. */
+typedef struct {
+    int mode, dtype;
+    uint64_t length, height, width;
+    const uint32_t *lags;
+    uint64_t num_lags;
+    uint64_t nx, ny, nt;
+    const void *window;
+    int window_f32;
+    int precision, out_dtype;
+    const void *d_img;
+    void *d_out;
+} fddm_batch_item;
+typedef struct fddm_batch fddm_batch;
+int fddm_batch_create(fddm_batch **batch, const fddm_batch_item *items, int num_items, int autotune,
+                      char *err, size_t errlen);
+int fddm_batch_launch(fddm_batch *batch, void *stream, char *err, size_t errlen);
+int fddm_batch_launch_eager(fddm_batch *batch, void *stream, char *err, size_t errlen);
+/* Frames per block of each item's plan (tb[num_items]). */
+int fddm_batch_frames_per_block(const fddm_batch *batch, int *tb);
+int fddm_batch_destroy(fddm_batch *batch);
+
 /* ---- FFT engine test entry: batch forward FFTs of length n, in place on
  * host data (interleaved double complex); precision FP64 or FP32. */
 int fddm_test_fft(double *data, uint64_t n, uint64_t batch, int precision, char *err,
