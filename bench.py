@@ -64,6 +64,20 @@ def run_gpu(core, img, lags, nx, ny, nt, win, precision, steps, warmup, keep_out
     stage_ms, n = plan.timing_end()
     stage_avg = {k: v / max(n, 1) for k, v in stage_ms.items()}
     out = d_out.to_array(plan.out_shape, np.float64) if keep_output else None
+    azavg_ms = None
+    if mode == "fft" and precision == "fp32c":
+        # azimuthal average (50 bins, with errors) of the device-resident result
+        from fastddm_amd.azimuthalaverage import azimuthal_average_device
+
+        kx = 2 * np.pi * np.fft.fftfreq(nx)[: nx // 2 + 1]
+        ky = 2 * np.pi * np.fft.fftshift(np.fft.fftfreq(ny))
+        tau = np.asarray(lags, dtype=np.float64)
+        azimuthal_average_device(d_out, plan.out_shape, kx, ky, tau, nx, np.float64, bins=50)
+        t = time.perf_counter()
+        for _ in range(3):
+            azimuthal_average_device(d_out, plan.out_shape, kx, ky, tau, nx, np.float64, bins=50)
+        azavg_ms = (time.perf_counter() - t) / 3 * 1e3
+    stage_avg["azimuthal_average_50_bins"] = azavg_ms
     for b in (d_img, d_out, d_ws):
         b.free()
     stream.destroy()

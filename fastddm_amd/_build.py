@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = ["fddm_kernels.hip", "fddm_time_p2.hip", "fddm_spatial_p2.hip", "fddm_capi.hip", "fddm_server.hip",
-           "fddm_batch.hip"]
+           "fddm_batch.hip", "fddm_azavg.hip"]
 TARGET = os.path.join(HERE, "libfddm_hip.so")
 
 
@@ -23,7 +23,7 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = True, jobs: int = 6) -> str:
+def build(verbose: bool = True, jobs: int = 7) -> str:
     """Compile every HIP source (in parallel) and link the shared library."""
     objs = []
     procs = []

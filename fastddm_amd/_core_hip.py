@@ -91,6 +91,7 @@ def lib() -> ctypes.CDLL:
         "fddm_batch_launch_eager": [p, p, cp, sz],
         "fddm_batch_frames_per_block": [p, ctypes.POINTER(i32)],
         "fddm_batch_destroy": [p],
+        "fddm_azimuthal_average": [p, i32, u64, u64, u64, p, p, p, i32, i32, p, p, p, cp, sz],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -142,6 +143,7 @@ EXPORTED_SYMBOLS = (
     "fddm_batch_launch_eager",
     "fddm_batch_frames_per_block",
     "fddm_batch_destroy",
+    "fddm_azimuthal_average",
 )
 
 

@@ -205,6 +205,22 @@ int fddm_batch_launch_eager(fddm_batch *batch, void *stream, char *err, size_t e
 int fddm_batch_frames_per_block(const fddm_batch *batch, int *tb);
 int fddm_batch_destroy(fddm_batch *batch);
 
+/* ---- azimuthal average of a device-resident result ----
+ * Replaces the reduction loops of azimuthal_average_array
+ * (src/python/azimuthalaverage.py:420-480; caller azimuthal_average :481-566).
+ * d_data: (rows, ny, nxh) float64 (data_f32 = 0) or float32 values in device
+ * memory, e.g. a plan output.  Per pixel q = y*nxh + x (host arrays):
+ * bin_index[q] in [0, nbins), non-decreasing along x; w[q] = counts*weights
+ * (0 excludes the pixel); w2[q] = counts*weights^2 (read when eval_err).
+ * Writes avg_out[nbins*rows] = sum w x / sum w and, when eval_err,
+ * err_out[nbins*rows] = sqrt(sum w (x - avg)^2 / (sum w - sum w2 / sum w)),
+ * both [bin][row] in float64 on the host; NaN for bins with no weight.
+ * nbins <= 1024.  stream: hipStream_t or NULL. */
+int fddm_azimuthal_average(const void *d_data, int data_f32, uint64_t rows, uint64_t ny,
+                           uint64_t nxh, const int32_t *bin_index, const double *w,
+                           const double *w2, int nbins, int eval_err, double *avg_out,
+                           double *err_out, void *stream, char *err, size_t errlen);
+
 /* ---- FFT engine test entry: batch forward FFTs of length n, in place on
  * host data (interleaved double complex); precision FP64 or FP32. */
 int fddm_test_fft(double *data, uint64_t n, uint64_t batch, int precision, char *err,

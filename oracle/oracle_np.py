@@ -107,3 +107,35 @@ def relative_error(x, ref):
     ref = np.asarray(ref, dtype=np.float64).ravel()
     m = ref != 0
     return np.abs(x[m] - ref[m]) / np.abs(ref[m])
+
+
+def azimuthal_average_array(data, dist, bin_edges, w, w2, eval_err=True):
+    """Weighted azimuthal average over pre-digitised pixels (FP64).
+
+    Restates azimuthal_average_array of the reference
+    (src/python/azimuthalaverage.py:420-480) for one flattened pixel set:
+    ``w``/``w2`` are counts*weights and counts*weights**2 with zeros for the
+    excluded pixels, bins come from ``np.searchsorted(bin_edges, dist)``.
+    Returns (avg, err) of shape (nbins, rows): avg = sum w x / sum w, NaN for
+    empty bins; err = sqrt(sum w (x - avg)^2 / (sum w - sum w2 / sum w)).
+    """
+    nb = len(bin_edges)
+    rows = data.shape[0]
+    x = data.reshape(rows, -1).astype(np.float64)
+    idx = np.minimum(np.searchsorted(bin_edges, dist.ravel()), nb - 1)
+    wf, w2f = w.ravel().astype(np.float64), w2.ravel().astype(np.float64)
+    sw = np.bincount(idx, weights=wf, minlength=nb)
+    sw2 = np.bincount(idx, weights=w2f, minlength=nb)
+    onehot = np.zeros((nb, idx.size))
+    onehot[idx, np.arange(idx.size)] = wf
+    with np.errstate(divide="ignore", invalid="ignore"):
+        avg = (onehot @ x.T) / sw[:, None]
+        err = None
+        if eval_err:
+            dev = (x.T[None, :, :] - avg[:, None, :]) ** 2          # (nb, npix, rows)
+            num = np.einsum("bp,bpr->br", onehot, dev)
+            bias = np.where(sw > 0, sw - sw2 / np.where(sw > 0, sw, 1.0), 0.0)
+            err = np.sqrt(num / bias[:, None])
+            err[sw == 0] = np.nan
+    avg[sw == 0] = np.nan
+    return avg, err
