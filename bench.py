@@ -194,6 +194,58 @@ def batch_main(args):
     }
 
 
+def precision_main(args):
+    """Error of D, PS and VAR against the FP64 oracle at the default workload
+    (2000 x 512^2 uint16 speckle, lags 1..1999, Blackman-Harris, nt = 4000)
+    for the fp64, fp32 and fp32c GPU paths: per lag, the median and max over q
+    of |x - x_ref| / |x_ref| (x_ref != 0), overall, and at q = 0.  x_ref is the
+    FP64 C restatement of the reference cpp core (oracle/ddm_port.c, OpenMP)."""
+    from fastddm_amd import _core_hip as core
+    from fastddm_amd.speckle import ar1_speckle
+    from fastddm_amd.window import blackman_harris
+    from oracle import port
+
+    if not os.path.isfile(port._LIB_PATH):
+        port.build()
+    L, S = 2000, 512
+    img = ar1_speckle(L, S, S, sigma=1.5, diffusivity=0.2, mean=1000.0, std=150.0, dtype=np.uint16, seed=2)
+    win = blackman_harris((S, S))
+    lags = np.arange(1, L)
+    threads = min(16, len(os.sched_getaffinity(0)))
+    t0 = time.perf_counter()
+    ref = port.ddm_fft(img, lags, S, S, 2 * L, win, chunk_size=257, threads=threads)
+    log(f"FP64 oracle ({threads} threads): {time.perf_counter() - t0:.1f} s")
+    core.set_device(0)
+    q0 = (S // 2) * (S // 2 + 1)  # q = 0 sits at row ny/2 after the fftshift
+    result = {"workload": "2000 x 512^2 uint16 speckle, lags 1..1999, Blackman-Harris, nt=4000, float64 out",
+              "reference": f"oracle/ddm_port.c FP64, {threads} threads", "paths": {}}
+    per_lag = {}
+    for prec in ("fp64", "fp32", "fp32c"):
+        out = core.ddm_fft(img, lags, S, S, 4000, win, precision=prec)
+        a, r = out.reshape(out.shape[0], -1), ref.reshape(ref.shape[0], -1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rel = np.abs(a - r) / np.abs(r)
+        rel[r == 0] = np.nan
+        med = np.nanmedian(rel, axis=1)
+        mx = np.nanmax(rel, axis=1)
+        per_lag[prec] = (med, mx)
+        d = rel[:-2]
+        result["paths"][prec] = {
+            "D_median": float(np.nanmedian(d)), "D_max": float(np.nanmax(d)),
+            "D_max_at_lag": int(lags[int(np.nanargmax(np.nanmax(d, axis=1)))]),
+            "PS_median": float(med[-2]), "PS_max": float(mx[-2]),
+            "VAR_median": float(med[-1]), "VAR_max": float(mx[-1]),
+            "q0_D_max": float(np.nanmax(d[:, q0])), "q0_D_median": float(np.nanmedian(d[:, q0])),
+            "per_lag_median_max": float(np.max(med[:-2])),
+        }
+        log(f"{prec}: {result['paths'][prec]}")
+        del out, a, rel, d
+    pick = [1, 2, 3, 5, 10, 20, 50, 100, 200, 500, 1000, 1500, 1900, 1968, 1990, 1999]
+    result["per_lag"] = {str(l): {p: [float(per_lag[p][0][l - 1]), float(per_lag[p][1][l - 1])] for p in per_lag}
+                         for l in pick}
+    return result
+
+
 def serve_main(args):
     """Serving run: Poisson arrivals of mixed analyses into the request server.
 
@@ -316,6 +368,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving measurement")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="only report fp64 / fp32 / fp32c errors against the FP64 oracle")
     ap.add_argument("--serve-requests", type=int, default=500)
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate (1/s)")
     ap.add_argument("--serve-slots", type=int, default=8)
@@ -332,6 +386,9 @@ def main():
         return
     if args.workload == "batch":
         print(json.dumps(batch_main(args)), flush=True)
+        return
+    if args.precision_report:
+        print(json.dumps(precision_main(args)), flush=True)
         return
     serve = None
     if not args.skip_serve and not args.copy_calibration and args.workload == "fft":
