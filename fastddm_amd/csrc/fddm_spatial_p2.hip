@@ -22,86 +22,93 @@ using namespace p2;
 // of resident workgroups, so each one brings 8 waves.
 constexpr int kThreads = 512;
 
+// Pixel (p, x) of a pair row: converted, frame 0 subtracted in integer
+// arithmetic where the type allows, times the window value.
 template <typename Tin, typename T>
-__device__ __forceinline__ T pixel(const Tin* __restrict__ img, size_t idx, size_t idx0, bool sub) {
+__device__ __forceinline__ T pixel_value(Tin v, Tin v0, bool sub, T w) {
   if constexpr (std::is_integral_v<Tin> && sizeof(Tin) <= 2) {
-    int v = static_cast<int>(img[idx]);
-    if (sub) v -= static_cast<int>(img[idx0]);
-    return static_cast<T>(v);
+    return static_cast<T>(static_cast<int>(v) - (sub ? static_cast<int>(v0) : 0)) * w;
   } else if constexpr (std::is_integral_v<Tin> && sizeof(Tin) == 4) {
-    long long v = static_cast<long long>(img[idx]);
-    if (sub) v -= static_cast<long long>(img[idx0]);
-    return static_cast<T>(v);
+    return static_cast<T>(static_cast<long long>(v) - (sub ? static_cast<long long>(v0) : 0)) * w;
   } else if constexpr (std::is_integral_v<Tin>) {
-    double v = static_cast<double>(img[idx]);
-    if (sub) v -= static_cast<double>(img[idx0]);
-    return static_cast<T>(v);
+    return static_cast<T>(static_cast<double>(v) - (sub ? static_cast<double>(v0) : 0.0)) * w;
   } else {
-    Tin v = img[idx];
-    if (sub) v -= img[idx0];
-    return static_cast<T>(v);
+    return static_cast<T>(sub ? v - v0 : v) * w;
   }
 }
 
 template <typename Tin, typename T, int LOGX, int NP>
 __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a) {
   using C = cplx_t<T>;
-  constexpr int NX = 1 << LOGX, NXH = NX / 2 + 1;
+  constexpr int NX = 1 << LOGX, NXH = NX / 2 + 1, TB = 2 * NP;
   constexpr SpPlan P = sp_plan(LOGX);
-  constexpr int XPT = (NX + kThreads - 1) / kThreads;  // x positions per thread
   extern __shared__ __align__(16) unsigned char smem[];
   C* const buf = reinterpret_cast<C*>(smem);
   const int tid = threadIdx.x;
-  const int H = a.H, W = a.W, Tb = a.Tb;
+  const int H = a.H, W = a.W;
   const int y = blockIdx.x % H;
   const uint64_t tb = blockIdx.x / H;
-  const uint64_t t0 = tb * Tb;
+  const uint64_t t0 = tb * TB;
   const Tin* __restrict__ img = static_cast<const Tin*>(a.img);
   const T* __restrict__ win = static_cast<const T*>(a.window);
   const size_t plane = static_cast<size_t>(H) * W;
   const bool sub = a.subtract0 != 0;
   const C* __restrict__ tw = static_cast<const C*>(a.twx);
-  constexpr int npairs = NP;  // == Tb / 2
+  const Tin* __restrict__ row0 = img + static_cast<size_t>(y) * W;  // frame 0, row y
 
-  // window values of this thread's x positions (the same for every pair)
-  T wv[XPT];
+  // pairs (2p, 2p+1) -> LDS, element (p, x) at logical p NX + x
+  if (W == NX && (static_cast<size_t>(W) * sizeof(Tin)) % 8 == 0) {
+    // full rows: one 8-byte load brings VX consecutive pixels
+    constexpr int VX = 8 / sizeof(Tin), G = NX / VX;
+    struct alignas(8) Vec {
+      Tin v[VX];
+    };
+    for (int e = tid; e < NP * G; e += kThreads) {
+      const int p = e / G, g = e - p * G, x0 = g * VX;
+      const uint64_t ta = t0 + 2 * p;
+      Vec va{}, vb{}, v0{};
+      if (ta < a.frames) va = *reinterpret_cast<const Vec*>(row0 + ta * plane + x0);
+      if (ta + 1 < a.frames) vb = *reinterpret_cast<const Vec*>(row0 + (ta + 1) * plane + x0);
+      if (sub) v0 = *reinterpret_cast<const Vec*>(row0 + x0);
+      C* d = buf + padi(p * NX + x0);  // VX divides 32: the VX slots are contiguous
 #pragma unroll
-  for (int i = 0; i < XPT; ++i) {
-    const int x = tid + i * kThreads;
-    wv[i] = (win && x < W) ? win[static_cast<size_t>(y) * W + x] : T(1);
-  }
-  // pairs -> LDS, element (p, x) at logical p NX + x
-  for (int p = 0; p < npairs; ++p) {
-    const uint64_t ta = t0 + 2 * p, tb2 = ta + 1;
-#pragma unroll
-    for (int i = 0; i < XPT; ++i) {
-      const int x = tid + i * kThreads;
-      if (x < NX) {
-        T va = 0, vb = 0;
-        if (x < W) {
-          const size_t i0 = static_cast<size_t>(y) * W + x;
-          if (ta < a.frames) va = pixel<Tin, T>(img, ta * plane + i0, i0, sub) * wv[i];
-          if (tb2 < a.frames) vb = pixel<Tin, T>(img, tb2 * plane + i0, i0, sub) * wv[i];
-        }
-        buf[padi(p * NX + x)] = C{va, vb};
+      for (int v = 0; v < VX; ++v) {
+        const T w = win ? win[static_cast<size_t>(y) * W + x0 + v] : T(1);
+        const T fa = ta < a.frames ? pixel_value<Tin, T>(va.v[v], v0.v[v], sub, w) : T(0);
+        const T fb = ta + 1 < a.frames ? pixel_value<Tin, T>(vb.v[v], v0.v[v], sub, w) : T(0);
+        d[v] = C{fa, fb};
       }
+    }
+  } else {
+    for (int e = tid; e < NP * NX; e += kThreads) {
+      const int p = e >> LOGX, x = e & (NX - 1);
+      const uint64_t ta = t0 + 2 * p;
+      T fa = 0, fb = 0;
+      if (x < W) {
+        const T w = win ? win[static_cast<size_t>(y) * W + x] : T(1);
+        const Tin x0v = sub ? row0[x] : Tin(0);
+        if (ta < a.frames) fa = pixel_value<Tin, T>(row0[ta * plane + x], x0v, sub, w);
+        if (ta + 1 < a.frames) fb = pixel_value<Tin, T>(row0[(ta + 1) * plane + x], x0v, sub, w);
+      }
+      buf[padi(e)] = C{fa, fb};
     }
   }
   __syncthreads();
   fft3_lds<T, NX, P.r0, P.r1, P.r2, kThreads, NP, false, 1, NX>(buf, tw, tid);
 
   // A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / 2i; store
-  // R[tb][k][y][ti] with ti fastest
+  // R[tb][k][y][ti] with ti fastest (runs of TB complex per k)
   const T half_norm = static_cast<T>(0.5 * a.norm);
-  C* __restrict__ R = static_cast<C*>(a.R);
-  for (int w = tid; w < NXH * Tb; w += kThreads) {
-    const int ti = w % Tb, k = w / Tb;
+  C* __restrict__ R = static_cast<C*>(a.R) + (tb * NXH * static_cast<size_t>(H) + y) * TB;
+  const uint32_t kstride = static_cast<uint32_t>(H) * TB;
+  for (int w = tid; w < NXH * TB; w += kThreads) {
+    const int ti = w & (TB - 1), k = w / TB;
     const int p = ti >> 1;
     const C zk = buf[padi(p * NX + k)];
     const C zn = buf[padi(p * NX + ((NX - k) & (NX - 1)))];
     const C o = (ti & 1) == 0 ? C{(zk.x + zn.x) * half_norm, (zk.y - zn.y) * half_norm}
                               : C{(zk.y + zn.y) * half_norm, (zn.x - zk.x) * half_norm};
-    R[((tb * NXH + k) * static_cast<size_t>(H) + y) * Tb + ti] = o;
+    R[static_cast<uint32_t>(k) * kstride + ti] = o;
   }
 }
 

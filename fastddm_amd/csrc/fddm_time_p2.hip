@@ -54,6 +54,21 @@ __device__ __forceinline__ A wave_incl_scan(A v) {
   return v;
 }
 
+// 1 / n for a positive integer n: hardware reciprocal refined by Newton's
+// iteration r <- r (2 - n r), twice for double (error below 2 ulp).
+template <typename A>
+__device__ __forceinline__ A recip(int n) {
+  if constexpr (sizeof(A) == 8) {
+    const double x = static_cast<double>(n);
+    double r = __builtin_amdgcn_rcp(x);
+    r = r * (2.0 - x * r);
+    r = r * (2.0 - x * r);
+    return r;
+  } else {
+    return 1.0f / static_cast<float>(n);
+  }
+}
+
 // Sums of x and y over the WPT waves of this thread's group (one barrier);
 // red holds two doubles per wave of the workgroup and is written once.
 template <int WPT>
@@ -146,14 +161,20 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs 
   const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(active ? q : 0) * a.Lp;
   const C* __restrict__ tw = static_cast<const C*>(a.twt);
 
-  // 1. first-pass inputs straight from HBM
+  // 1. first-pass inputs straight from HBM (clamped indices, no branches;
+  // samples t >= L are replaced by the zero padding afterwards)
   C v0[MB0][R0];
 #pragma unroll
   for (int b = 0; b < MB0; ++b)
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
       const int t = u + b * TPT + r * M0;
-      v0[b][r] = (r < RH && active && t < L) ? src[t] : C{0, 0};
+      if (r < RH) {
+        const C f = src[min(t, L - 1)];
+        v0[b][r] = (active && t < L) ? f : C{0, 0};
+      } else {
+        v0[b][r] = C{0, 0};
+      }
     }
 
   // 2. temporal mean (the compensated paths transform F - mean)
@@ -166,7 +187,8 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs 
       si += static_cast<double>(v0[b][r].y);
     }
   group_sum2<WPT>(sr, si, red);
-  const double mr = sr / L, mi = si / L;
+  const double rl = recip<double>(L);
+  const double mr = sr * rl, mi = si * rl;
   const T m_r = SUB ? static_cast<T>(mr) : T(0), m_i = SUB ? static_cast<T>(mi) : T(0);
   if constexpr (SUB) {
 #pragma unroll
@@ -266,7 +288,9 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs 
   const int ky = q / a.nxh, kx = q - ky * a.nxh;
   const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
   const Acc inv_n = static_cast<Acc>(1) / static_cast<Acc>(N);
-  for (int i = u; i < a.nl; i += TPT) {
+  // lags in reverse order: the largest lags (direct sums, longest loops) all
+  // fall in the first wave of the group
+  for (int i = a.nl - 1 - u; i >= 0; i -= TPT) {
     const int d = static_cast<int>(a.lags[i]);
     Acc D;
     if (SUB && L - d <= kEdge) {
@@ -279,12 +303,12 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs 
         const Acc di = static_cast<Acc>(x.y) - static_cast<Acc>(y.y);
         acc += dr * dr + di * di;
       }
-      D = acc / static_cast<Acc>(L - d);
+      D = acc * recip<Acc>(L - d);
     } else {
       const Acc head = tail[L - d - 1];   // psum(L - d)
       const Acc tl = ptot - tail[d - 1];  // psum(L) - psum(d)
       const Acc c = static_cast<Acc>(buf[padi(d)].x) * inv_n;
-      D = (head + tl - 2 * c) / static_cast<Acc>(L - d);
+      D = (head + tl - 2 * c) * recip<Acc>(L - d);
     }
     out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(D);
   }
