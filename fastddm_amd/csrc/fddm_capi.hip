@@ -150,11 +150,13 @@ std::vector<T> twiddles(const FftDesc& d) {
   return w;
 }
 
-// Twiddles of the power-of-two time plan (p2_plan): FFT1 passes 1 .. np-1,
-// then FFT2 passes 1 .. np-1, each [k][r] = exp(-2 pi i r k / (NS R)).
+// Twiddles of the power-of-two time plans: passes 1 .. np-1 of p2_plan, then
+// of p2_half_plan (length N/2), each [k][r] = exp(-2 pi i r k / (NS R)), then
+// (cos, sin)(2 pi k / N) for k < N/2.
 template <typename T>
 std::vector<T> p2_twiddles(int logn) {
-  const P2Plan P = p2_plan(logn, sizeof(T) == 8);
+  const bool f64 = sizeof(T) == 8;
+  const P2Plan P = p2_plan(logn, f64), Q = p2_half_plan(logn, f64);
   std::vector<T> w;
   const long double two_pi = 6.283185307179586476925286766559005768L;
   auto pass = [&](long long R, long long NS) {
@@ -166,7 +168,13 @@ std::vector<T> p2_twiddles(int logn) {
       }
   };
   for (int i = 1; i < P.np; ++i) pass(p2_radix(P, i), p2_ns(P, i));
-  for (int i = 1; i < P.np; ++i) pass(p2_radix_b(P, i), p2_ns_b(P, i));
+  for (int i = 1; i < Q.np; ++i) pass(p2_radix(Q, i), p2_ns(Q, i));
+  const long long n = 1ll << logn;
+  for (long long k = 0; k < n / 2; ++k) {
+    const long double ang = two_pi * static_cast<long double>(k) / static_cast<long double>(n);
+    w.push_back(static_cast<T>(cosl(ang)));
+    w.push_back(static_cast<T>(sinl(ang)));
+  }
   return w;
 }
 

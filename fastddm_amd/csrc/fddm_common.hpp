@@ -48,7 +48,7 @@ __host__ __device__ constexpr int lds_padded_size(int n) { return n + (n >> 5) +
 // transform runs them in reverse).  Every pass has a whole number of
 // butterflies per thread: N / (R * T) >= 1.
 // Time-axis plans for N = 2^logn: threads per transform, number of passes
-// (3 or 4) and the radices of FFT1 in order (FFT2 runs them reversed).  For
+// (3 or 4) and the radices of the first transform in order.  For
 // FP32 at N >= 4096 the transform is spread over 512 or 1024 threads (radix
 // 8 or 16 passes) so that a CU holds 32 (N = 4096) or 16 waves despite the
 // LDS buffer; FP64 keeps three passes on 256 threads, which measured faster
@@ -67,23 +67,36 @@ __host__ __device__ constexpr P2Plan p2_plan(int logn, bool f64) {
 __host__ __device__ constexpr int p2_radix(const P2Plan& P, int i) {
   return i == 0 ? P.r0 : i == 1 ? P.r1 : i == 2 ? P.r2 : P.r3;
 }
-// Radix of pass i of FFT2 (the FFT1 radices reversed).
-__host__ __device__ constexpr int p2_radix_b(const P2Plan& P, int i) { return p2_radix(P, P.np - 1 - i); }
+// Plan of the second transform: |G|^2 is real, so its length-N transform
+// runs as one length-N/2 complex transform of z(n) = P(2n) + i P(2n+1) on the
+// same threads (passes with fewer butterflies than threads leave whole waves
+// idle), followed by a per-lag untangling step.
+__host__ __device__ constexpr P2Plan p2_half_plan(int logn, bool f64) {
+  return logn == 9    ? P2Plan{64, 3, 8, 8, 4, 1}
+         : logn == 10 ? P2Plan{64, 3, 8, 8, 8, 1}
+         : logn == 11 ? P2Plan{128, 3, 16, 16, 4, 1}
+         : logn == 12 ? (f64 ? P2Plan{256, 3, 16, 16, 8, 1} : P2Plan{512, 4, 8, 8, 8, 4})
+         : logn == 13 ? (f64 ? P2Plan{256, 3, 16, 16, 16, 1} : P2Plan{512, 4, 8, 8, 8, 8})
+                      : P2Plan{1024, 4, 16, 8, 8, 8};
+}
 // Points already combined before pass i (product of the earlier radices).
 __host__ __device__ constexpr int p2_ns(const P2Plan& P, int i) { return i == 0 ? 1 : p2_ns(P, i - 1) * p2_radix(P, i - 1); }
-__host__ __device__ constexpr int p2_ns_b(const P2Plan& P, int i) {
-  return i == 0 ? 1 : p2_ns_b(P, i - 1) * p2_radix_b(P, i - 1);
+// Twiddle table layout (complex entries): FFT1 passes 1 .. np-1, then the
+// half-length passes 1 .. np-1 of the second transform, then N/2 entries
+// (cos, sin)(2 pi k / N) for the untangling step.  The pass with radix R
+// after NS points holds exp(-2 pi i r k / (NS R)) at offset k R + r of its
+// table (NS R entries).
+__host__ __device__ constexpr int p2_tw_pass(const P2Plan& P, int i, int base) {
+  return i <= 1 ? base : p2_tw_pass(P, i - 1, base) + p2_ns(P, i - 1) * p2_radix(P, i - 1);
 }
-// Twiddle table layout (complex entries): FFT1 passes 1 .. np-1, then FFT2
-// passes 1 .. np-1; the pass with radix R after NS points holds
-// exp(-2 pi i r k / (NS R)) at offset k R + r of its table (NS R entries).
-__host__ __device__ constexpr int p2_tw_a(const P2Plan& P, int i) {
-  return i <= 1 ? 0 : p2_tw_a(P, i - 1) + p2_ns(P, i - 1) * p2_radix(P, i - 1);
+__host__ __device__ constexpr int p2_tw_a(int logn, bool f64, int i) { return p2_tw_pass(p2_plan(logn, f64), i, 0); }
+__host__ __device__ constexpr int p2_tw_h(int logn, bool f64, int i) {
+  return p2_tw_pass(p2_half_plan(logn, f64), i, p2_tw_a(logn, f64, p2_plan(logn, f64).np));
 }
-__host__ __device__ constexpr int p2_tw_b(const P2Plan& P, int i) {
-  return i <= 1 ? p2_tw_a(P, P.np) : p2_tw_b(P, i - 1) + p2_ns_b(P, i - 1) * p2_radix_b(P, i - 1);
+__host__ __device__ constexpr int p2_tw_post(int logn, bool f64) {
+  return p2_tw_h(logn, f64, p2_half_plan(logn, f64).np);
 }
-__host__ __device__ constexpr int p2_tw_total(const P2Plan& P) { return p2_tw_b(P, P.np); }
+__host__ __device__ constexpr int p2_tw_total(int logn, bool f64) { return p2_tw_post(logn, f64) + (1 << (logn - 1)); }
 constexpr int kP2MinLog = 9, kP2MaxLog = 14;
 
 template <typename T> struct Cplx;

@@ -14,11 +14,11 @@
 //      HBM (t >= N/2 >= L is the zero padding and is never read);
 //   2. compensated paths: subtracts the temporal mean (one group reduction);
 //   3. FFT1 (radices R0 .. RL); the last pass stays in registers, |G|^2 is
-//      formed there and FFT2's first pass (radix RL, same distribution) runs
-//      on it;
-//   4. FFT2 (RL .. R0); the last pass leaves Re C(t) (times N) in the same
-//      registers as the loaded F(t);
-//   5. writes C(t) to LDS, re-reads the series (cache resident) for the
+//      formed there and written to LDS as z(n) = P(2n) + i P(2n+1);
+//   4. the second forward transform of the real P runs as one complex
+//      transform of length N/2 of z (p2_half_plan), left in LDS; for each lag
+//      Re FFT_N(P)(d) = N C(d) is untangled from Z(d) and Z(N/2 - d);
+//   5. re-reads the series (cache resident) for the
 //      prefix sums of |F(t)|^2 (double on the compensated paths; contiguous
 //      segments per thread and one group scan) and keeps the first and last
 //      kEdge samples; then evaluates D for every requested lag (direct sums
@@ -108,23 +108,25 @@ struct P2Lds {
   static constexpr size_t bytes = buf + edge + red + chunk;
 };
 
-// Middle passes (LDS to LDS) of one transform: FFT1 (B = false) passes
-// I .. np-2, or FFT2 (B = true) passes I .. np-2.
-template <typename T, int LOGN, bool B, int I>
-__device__ __forceinline__ void p2_mid_passes(cplx_t<T>* buf, const cplx_t<T>* __restrict__ tw, int u) {
-  constexpr P2Plan P = p2_plan(LOGN, sizeof(T) == 8);
-  if constexpr (I < P.np - 1) {
-    constexpr int N = 1 << LOGN;
-    constexpr int R = B ? p2_radix_b(P, I) : p2_radix(P, I);
-    constexpr int NS = B ? p2_ns_b(P, I) : p2_ns(P, I);
-    constexpr int OFF = B ? p2_tw_b(P, I) : p2_tw_a(P, I);
+// LDS-to-LDS passes I .. LAST-1 of a transform (HALF = false: the first
+// transform, plan p2_plan, length 2^LOGN; HALF = true: the half-length second
+// transform, plan p2_half_plan, length 2^(LOGN-1)).
+template <typename T, int LOGN, bool HALF, int I, int LAST>
+__device__ __forceinline__ void p2_passes(cplx_t<T>* buf, const cplx_t<T>* __restrict__ tw, int u) {
+  constexpr bool F64 = sizeof(T) == 8;
+  constexpr P2Plan P = HALF ? p2_half_plan(LOGN, F64) : p2_plan(LOGN, F64);
+  if constexpr (I < LAST) {
+    constexpr int N = HALF ? 1 << (LOGN - 1) : 1 << LOGN;
+    constexpr int R = p2_radix(P, I);
+    constexpr int NS = p2_ns(P, I);
+    constexpr int OFF = HALF ? p2_tw_h(LOGN, F64, I) : p2_tw_a(LOGN, F64, I);
     using Ps = Pass<T, N, R, NS, P.threads>;
     cplx_t<T> v[Ps::MB][R];
     Ps::compute(buf, v, tw + OFF, u);
     __syncthreads();
     Ps::store(buf, v, u);
     __syncthreads();
-    p2_mid_passes<T, LOGN, B, I + 1>(buf, tw, u);
+    p2_passes<T, LOGN, HALF, I + 1, LAST>(buf, tw, u);
   }
 }
 
@@ -135,14 +137,12 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs 
   using Acc = typename Lay::Acc;
   constexpr P2Plan P = Lay::P;
   constexpr int N = 1 << LOGN, TPT = P.threads, G = Lay::G, WPT = Lay::WPT, NP = P.np;
-  constexpr int R0 = P.r0, RL = p2_radix(P, NP - 1);
-  using A0 = Pass<T, N, R0, 1, TPT>;                          // FFT1 first pass (registers)
-  using AL = Pass<T, N, RL, N / RL, TPT>;                     // FFT1 last pass (to registers)
-  using B0 = Pass<T, N, RL, 1, TPT>;                          // FFT2 first pass (registers)
-  using BL = Pass<T, N, R0, N / R0, TPT>;                     // FFT2 last pass (to registers)
-  static_assert(AL::MB == B0::MB && BL::MB == A0::MB, "register hand-offs");
+  constexpr bool F64 = sizeof(T) == 8;
+  constexpr int R0 = P.r0, RL = p2_radix(P, NP - 1), NH = N / 2;
+  using A0 = Pass<T, N, R0, 1, TPT>;                          // first pass (registers)
+  using AL = Pass<T, N, RL, N / RL, TPT>;                     // last pass (to registers)
   static_assert(N % (R0 * TPT) == 0, "whole first-pass butterflies per thread");
-  constexpr int TWA = p2_tw_a(P, NP - 1), TWB = p2_tw_b(P, NP - 1);
+  constexpr int TWA = p2_tw_a(LOGN, F64, NP - 1), TWP = p2_tw_post(LOGN, F64);
   constexpr int MB0 = A0::MB, M0 = A0::M, RH = R0 / 2;
 
   extern __shared__ __align__(16) unsigned char smem[];
@@ -206,38 +206,34 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs 
   const double fsr = sr - static_cast<double>(L) * static_cast<double>(m_r);
   const double fsi = si - static_cast<double>(L) * static_cast<double>(m_i);
 
-  // 3. FFT1: first pass on the registers, last pass back into registers
+  // 3. first transform G = FFT(F): first pass on the registers, last pass
+  // back into registers; P = |G|^2 goes to LDS as z(n) = P(2n) + i P(2n+1)
   A0::compute_regs(v0);
   A0::store(buf, v0, u);
   __syncthreads();
-  p2_mid_passes<T, LOGN, false, 1>(buf, tw, u);
-  C v2[AL::MB][RL];
-  AL::compute(buf, v2, tw + TWA, u);
+  p2_passes<T, LOGN, false, 1, NP - 1>(buf, tw, u);
+  {
+    C v2[AL::MB][RL];
+    AL::compute(buf, v2, tw + TWA, u);
+    __syncthreads();
+    T* const zf = reinterpret_cast<T*>(buf);
 #pragma unroll
-  for (int b = 0; b < AL::MB; ++b)
+    for (int b = 0; b < AL::MB; ++b)
 #pragma unroll
-    for (int r = 0; r < RL; ++r) v2[b][r] = C{v2[b][r].x * v2[b][r].x + v2[b][r].y * v2[b][r].y, 0};
-  // 4. FFT2 of |G|^2, same register hand-offs
-  B0::compute_regs(v2);
+      for (int r = 0; r < RL; ++r) {
+        const int w = u + b * TPT + r * (N / RL);  // output position of the last pass
+        zf[2 * padi(w >> 1) + (w & 1)] = v2[b][r].x * v2[b][r].x + v2[b][r].y * v2[b][r].y;
+      }
+  }
   __syncthreads();
-  B0::store(buf, v2, u);
-  __syncthreads();
-  p2_mid_passes<T, LOGN, true, 1>(buf, tw, u);
-  BL::compute(buf, v0, tw + TWB, u);
-  __syncthreads();
+  // 4. second transform at half length: Z = FFT_{N/2}(z), left in LDS in
+  // natural order (elements 0 .. N/2-1 of the buffer)
+  p2_passes<T, LOGN, true, 0, p2_half_plan(LOGN, F64).np>(buf, tw, u);
 
-  // 5. C(t) N into LDS; prefix sums psum(t + 1) of |F(t) - mean|^2 from a
-  // second (cache-resident) read of the series, in chunks of T consecutive t
-  // (wave scans combined across the group's waves), into the free tail of the
+  // 5. prefix sums psum(t + 1) of |F(t) - mean|^2 from a second
+  // (cache-resident) read of the series into the free second half of the
   // buffer; first and last kEdge samples into `edge`.
-  Acc* const tail = reinterpret_cast<Acc*>(buf + padi(L));
-#pragma unroll
-  for (int b = 0; b < MB0; ++b)
-#pragma unroll
-    for (int r = 0; r < RH; ++r) {
-      const int t = u + b * TPT + r * M0;
-      if (t < L) buf[padi(t)].x = v0[b][r].x;
-    }
+  Acc* const tail = reinterpret_cast<Acc*>(buf + padi(NH));
   // each thread: a contiguous segment of seg <= SEGMAX samples, local running
   // sums, then one exclusive scan over the group's threads; seg is odd, so
   // the 8-byte stores of neighbouring threads fall in different LDS banks
@@ -307,7 +303,12 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs 
     } else {
       const Acc head = tail[L - d - 1];   // psum(L - d)
       const Acc tl = ptot - tail[d - 1];  // psum(L) - psum(d)
-      const Acc c = static_cast<Acc>(buf[padi(d)].x) * inv_n;
+      // N C(d) = Re FFT_N(P)(d) from A = Z(d), B = Z(N/2 - d) and
+      // (c, s) = (cos, sin)(2 pi d / N): 2 Re X = A.x + B.x + c (A.y + B.y) - s (A.x - B.x)
+      const C za = buf[padi(d)], zb = buf[padi((NH - d) & (NH - 1))], cs = tw[TWP + d];
+      const Acc ax = za.x, ay = za.y, bx = zb.x, by = zb.y;
+      const Acc re = (ax + bx) + static_cast<Acc>(cs.x) * (ay + by) - static_cast<Acc>(cs.y) * (ax - bx);
+      const Acc c = re * (Acc(0.5) * inv_n);
       D = (head + tl - 2 * c) * recip<Acc>(L - d);
     }
     out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(D);
