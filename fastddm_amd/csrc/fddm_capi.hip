@@ -11,8 +11,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "fddm_hip.h"
@@ -483,7 +485,9 @@ int plan_execute(fddm_plan* p, const void* d_img, void* d_out, void* ws, hipStre
   return FDDM_OK;
 }
 
-// Device buffers reused across the host-buffer entry points, per device.
+// Device buffers reused across the host-buffer entry points, per device,
+// and two pinned staging buffers for the host link.
+constexpr size_t kStageBytes = size_t(64) << 20;
 struct HostPathCache {
   void* img = nullptr;
   uint64_t img_cap = 0;
@@ -492,7 +496,69 @@ struct HostPathCache {
   void* ws = nullptr;
   uint64_t ws_cap = 0;
   hipStream_t stream = nullptr;
+  void* pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
 };
+
+// memcpy split over host threads (pageable <-> pinned staging); the caller's
+// pages are touched by several threads at once.
+void parallel_memcpy(void* dst, const void* src, size_t n) {
+  const size_t per = size_t(4) << 20;
+  unsigned nt = static_cast<unsigned>(std::min<size_t>((n + per - 1) / per, 8));
+  const unsigned hw = std::thread::hardware_concurrency();
+  if (hw && nt > hw) nt = hw;
+  if (nt <= 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t slice = (n + nt - 1) / nt;
+  for (unsigned i = 1; i < nt; ++i) {
+    const size_t b = i * slice, e = std::min(n, b + slice);
+    if (b < e)
+      th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + b, static_cast<const char*>(src) + b, e - b); });
+  }
+  std::memcpy(dst, src, std::min(n, slice));
+  for (auto& t : th) t.join();
+}
+
+// Host -> device through the staging pair: host threads fill one buffer while
+// the DMA engine drains the other.
+int h2d_staged(HostPathCache& c, void* d_dst, const void* h_src, size_t n, char* err, size_t errlen) {
+  size_t off = 0;
+  for (int k = 0; off < n; k ^= 1) {
+    const size_t m = std::min(kStageBytes, n - off);
+    FDDM_HIP(hipEventSynchronize(c.ev[k]));
+    parallel_memcpy(c.pin[k], static_cast<const char*>(h_src) + off, m);
+    FDDM_HIP(hipMemcpyAsync(static_cast<char*>(d_dst) + off, c.pin[k], m, hipMemcpyHostToDevice, c.stream));
+    FDDM_HIP(hipEventRecord(c.ev[k], c.stream));
+    off += m;
+  }
+  return FDDM_OK;
+}
+
+// Device -> host: two chunks in flight; each finished chunk is copied out by
+// host threads while the next one transfers.
+int d2h_staged(HostPathCache& c, void* h_dst, const void* d_src, size_t n, char* err, size_t errlen) {
+  const size_t nchunk = (n + kStageBytes - 1) / kStageBytes;
+  auto issue = [&](size_t i) -> int {
+    const size_t off = i * kStageBytes, m = std::min(kStageBytes, n - off);
+    FDDM_HIP(hipMemcpyAsync(c.pin[i & 1], static_cast<const char*>(d_src) + off, m, hipMemcpyDeviceToHost,
+                            c.stream));
+    FDDM_HIP(hipEventRecord(c.ev[i & 1], c.stream));
+    return FDDM_OK;
+  };
+  int rc;
+  for (size_t i = 0; i < std::min<size_t>(2, nchunk); ++i)
+    if ((rc = issue(i))) return rc;
+  for (size_t i = 0; i < nchunk; ++i) {
+    const size_t off = i * kStageBytes, m = std::min(kStageBytes, n - off);
+    FDDM_HIP(hipEventSynchronize(c.ev[i & 1]));
+    parallel_memcpy(static_cast<char*>(h_dst) + off, c.pin[i & 1], m);
+    if (i + 2 < nchunk && (rc = issue(i + 2))) return rc;
+  }
+  return FDDM_OK;
+}
 std::mutex g_cache_mu;
 HostPathCache g_cache[64];
 
@@ -538,12 +604,16 @@ int host_path(int mode, const void* img, int dtype, uint64_t L, uint64_t H, uint
                    "Not enough space on GPU: %llu bytes needed, %llu free",
                    static_cast<unsigned long long>(need), static_cast<unsigned long long>(free_b));
   if (!c.stream) FDDM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  for (int k = 0; k < 2; ++k) {
+    if (!c.pin[k]) FDDM_HIP(hipHostMalloc(&c.pin[k], kStageBytes, hipHostMallocDefault));
+    if (!c.ev[k]) FDDM_HIP(hipEventCreateWithFlags(&c.ev[k], hipEventDisableTiming));
+  }
   if ((rc = grow(&c.img, &c.img_cap, img_bytes, err, errlen))) return rc;
   if ((rc = grow(&c.out, &c.out_cap, out_b, err, errlen))) return rc;
   if ((rc = grow(&c.ws, &c.ws_cap, p->ws_bytes, err, errlen))) return rc;
-  FDDM_HIP(hipMemcpyAsync(c.img, img, img_bytes, hipMemcpyHostToDevice, c.stream));
+  if ((rc = h2d_staged(c, c.img, img, img_bytes, err, errlen))) return rc;
   if ((rc = plan_execute(p, c.img, c.out, c.ws, c.stream, err, errlen))) return rc;
-  FDDM_HIP(hipMemcpyAsync(out, c.out, out_b, hipMemcpyDeviceToHost, c.stream));
+  if ((rc = d2h_staged(c, out, c.out, out_b, err, errlen))) return rc;
   FDDM_HIP(hipStreamSynchronize(c.stream));
   return FDDM_OK;
 }
@@ -612,6 +682,10 @@ int fddm_release_cache(char* err, size_t errlen) {
   if (c.stream) FDDM_HIP(hipStreamSynchronize(c.stream));
   for (void* ptr : {c.img, c.out, c.ws})
     if (ptr) (void)hipFree(ptr);
+  for (int k = 0; k < 2; ++k) {
+    if (c.pin[k]) (void)hipHostFree(c.pin[k]);
+    if (c.ev[k]) (void)hipEventDestroy(c.ev[k]);
+  }
   if (c.stream) (void)hipStreamDestroy(c.stream);
   c = HostPathCache{};
   return FDDM_OK;
