@@ -8,8 +8,7 @@ all fftshifted along y. ``kx`` is the non-negative half axis, ``ky`` the full
 shifted axis, ``tau`` the lags. ``full_*`` rebuilds the symmetric full plane
 from the half plane (reference ``_reconstruct_full_spectrum``, ``:887-944``).
 
-The ``.sf.ddm`` binary writer/reader of the reference (``:409-884``) is not
-part of this package.
+``save`` writes the reference's ``.sf.ddm`` binary format (``fastddm_amd.sfio``).
 """
 
 from dataclasses import dataclass
@@ -110,6 +109,13 @@ class ImageStructureFunction:
 
     def set_frame_rate(self, frame_rate: float) -> None:
         self.delta_t = 1 / frame_rate
+
+    def save(self, fname: str = "analysis_blob") -> str:
+        """Write to ``fname`` (``.sf.ddm`` appended when missing) in the
+        reference's binary format (imagestructurefunction.py:279-293)."""
+        from .sfio import save
+
+        return save(self, fname)
 
     def full_shape(self) -> Tuple[int, int, int]:
         dim_t, dim_y = self.shape[:-1]
