@@ -11,6 +11,6 @@ from ._ddm import ddm, _backend  # noqa: F401
 from ._fftopt import next_fast_len  # noqa: F401
 from .imagestructurefunction import ImageStructureFunction  # noqa: F401
 from .azimuthalaverage import AzimuthalAverage, azimuthal_average  # noqa: F401
-from . import lags, window  # noqa: F401
+from . import intermediatescatteringfunction, lags, noise_est, sfio, window  # noqa: F401
 
 __version__ = "0.1.0"
