@@ -15,7 +15,9 @@ the q grid); the (rows, ny, nxh) data are reduced by the HIP kernels of
 """
 
 import ctypes
+import hashlib
 import warnings
+from collections import OrderedDict
 from dataclasses import dataclass
 from typing import Iterable, Optional, Tuple, Union
 
@@ -163,20 +165,59 @@ def _reduce(ptr, f32, rows, ny, nxh, bin_index, w, w2, nbins, eval_err):
     return avg, err
 
 
+_MAPS_CACHE: "OrderedDict[bytes, tuple]" = OrderedDict()
+_MAPS_CACHE_SIZE = 8
+
+
+def _maps_key(kx, ky, width, bins, range, mask, weights, eval_err) -> Optional[bytes]:
+    """Digest of everything the bin maps depend on; None (not cached) for a
+    non-boolean mask, whose cast warns on every call."""
+    if mask is not None and np.asarray(mask).dtype != bool:
+        return None
+    h = hashlib.blake2b(digest_size=20)
+    for a in (kx, ky, mask, weights, None if isinstance(bins, (int, np.integer)) else bins):
+        if a is None:
+            h.update(b"-")
+        else:
+            a = np.ascontiguousarray(a)
+            h.update(f"{a.dtype.str}{a.shape}".encode())
+            h.update(a.tobytes())
+    h.update(repr((int(width), bins if isinstance(bins, (int, np.integer)) else "edges",
+                   None if range is None else tuple(float(r) for r in range), bool(eval_err))).encode())
+    return h.digest()
+
+
+def _device_maps(kx, ky, width, ny, nxh, bins, range, mask, weights, eval_err):
+    """Bin maps of the half-plane (ny, nxh) grid, kept for the last few
+    geometries: repeated averages of results of one size reuse them."""
+    key = _maps_key(kx, ky, width, bins, range, mask, weights, eval_err)
+    if key is not None and key in _MAPS_CACHE:
+        _MAPS_CACHE.move_to_end(key)
+        return _MAPS_CACHE[key]
+    counts = _half_plane_counts((ny, nxh), width)
+    maps = bin_maps(_k_modulus(kx, ky), bins, range, mask, weights, counts, eval_err)
+    if np.any(np.diff(maps[0], axis=1) < 0):
+        raise ValueError("bin indices must not decrease along kx")
+    for a in maps:
+        a.setflags(write=False)
+    if key is not None:
+        _MAPS_CACHE[key] = maps
+        while len(_MAPS_CACHE) > _MAPS_CACHE_SIZE:
+            _MAPS_CACHE.popitem(last=False)
+    return maps
+
+
 def azimuthal_average_device(d_data, shape, kx, ky, tau, width, data_dtype=np.float64, bins=10, range=None,
                              mask=None, weights=None, eval_err=True) -> AzimuthalAverage:
     """Azimuthal average of a (NL+2, ny, nxh) result held in device memory
     (a ``DeviceBuffer`` or a raw device pointer), e.g. a plan's output or a
     served request's result."""
     rows, ny, nxh = shape
-    counts = _half_plane_counts((ny, nxh), width)
-    bi, w, w2, edges, k = bin_maps(_k_modulus(kx, ky), bins, range, mask, weights, counts, eval_err)
-    if np.any(np.diff(bi, axis=1) < 0):
-        raise ValueError("bin indices must not decrease along kx")
+    bi, w, w2, edges, k = _device_maps(kx, ky, width, ny, nxh, bins, range, mask, weights, eval_err)
     ptr = d_data.ptr if hasattr(d_data, "ptr") else ctypes.c_void_p(d_data)
     avg, err = _reduce(ptr, np.dtype(data_dtype) == np.float32, rows, ny, nxh, bi, w, w2, len(edges), eval_err)
-    return AzimuthalAverage(avg.astype(DTYPE), None if err is None else err.astype(DTYPE), k, np.asarray(tau),
-                            edges)
+    return AzimuthalAverage(avg.astype(DTYPE), None if err is None else err.astype(DTYPE), k.copy(), np.asarray(tau),
+                            edges.copy())
 
 
 def azimuthal_average(img_str_func: ImageStructureFunction, bins: Optional[Union[int, Iterable[float]]] = 10,

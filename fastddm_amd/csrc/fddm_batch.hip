@@ -74,20 +74,19 @@ int time_candidate(const fddm_batch_item& it, uint64_t frames, int tb, hipStream
   }
   void* d_ws = buf;
   void* d_out = static_cast<unsigned char*>(buf) + align256(ws);
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  rc = fddm_plan_execute(p, it.d_img, d_out, d_ws, s, err, errlen);  // warm-up
-  if (!rc) {
-    hipEventRecord(e0, s);
-    for (int i = 0; i < 3 && !rc; ++i) rc = fddm_plan_execute(p, it.d_img, d_out, d_ws, s, err, errlen);
-    hipEventRecord(e1, s);
-    hipEventSynchronize(e1);
-    hipEventElapsedTime(ms, e0, e1);
-  }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(buf);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = set_err(err, errlen, FDDM_ERR_HIP, "autotune events");
+  if (!rc) rc = fddm_plan_execute(p, it.d_img, d_out, d_ws, s, err, errlen);  // warm-up
+  if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = set_err(err, errlen, FDDM_ERR_HIP, "autotune event record");
+  for (int i = 0; i < 3 && !rc; ++i) rc = fddm_plan_execute(p, it.d_img, d_out, d_ws, s, err, errlen);
+  if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+    rc = set_err(err, errlen, FDDM_ERR_HIP, "autotune timing");
+  (void)hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(buf);
   fddm_plan_destroy(p);
   return rc;
 }
@@ -136,12 +135,12 @@ struct fddm_batch {
   hipGraphExec_t exec = nullptr;
 
   void release() {
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
-    for (hipEvent_t e : events) hipEventDestroy(e);
-    for (hipStream_t s : branches) hipStreamDestroy(s);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    for (hipStream_t s : branches) (void)hipStreamDestroy(s);
     for (fddm_plan* p : plans) fddm_plan_destroy(p);
-    if (ws_base) hipFree(ws_base);
+    if (ws_base) (void)hipFree(ws_base);
     exec = nullptr;
     graph = nullptr;
     events.clear();
@@ -172,14 +171,14 @@ int fddm_batch_create(fddm_batch** out, const fddm_batch_item* items, int n, int
   for (int i = 0; i < n; ++i) {
     const fddm_batch_item& it = items[i];
     if (!it.d_img || !it.d_out) {
-      hipStreamDestroy(tune_stream);
+      (void)hipStreamDestroy(tune_stream);
       return fail(set_err(err, errlen, FDDM_ERR_ARG, "item %d: NULL device pointer", i));
     }
     int tb = 0;
     if (autotune) {
       const int rc = tuned_frames_per_block(it, tune_stream, &tb, err, errlen);
       if (rc) {
-        hipStreamDestroy(tune_stream);
+        (void)hipStreamDestroy(tune_stream);
         return fail(rc);
       }
     }
@@ -188,7 +187,7 @@ int fddm_batch_create(fddm_batch** out, const fddm_batch_item* items, int n, int
                                         it.num_lags, it.nx, it.ny, it.nt, it.window, it.window_f32, it.precision,
                                         it.out_dtype, tb, err, errlen);
     if (rc) {
-      hipStreamDestroy(tune_stream);
+      (void)hipStreamDestroy(tune_stream);
       return fail(rc);
     }
     b->plans.push_back(p);
@@ -196,7 +195,7 @@ int fddm_batch_create(fddm_batch** out, const fddm_batch_item* items, int n, int
     fddm_plan_workspace_size(p, &ws);
     total += align256(ws);
   }
-  hipStreamDestroy(tune_stream);
+  (void)hipStreamDestroy(tune_stream);
   if (hipMalloc(&b->ws_base, std::max<uint64_t>(total, 256)) != hipSuccess)
     return fail(set_err(err, errlen, FDDM_ERR_MEMORY, "batch workspace of %llu bytes", (unsigned long long)total));
   uint64_t off = 0;
@@ -229,18 +228,20 @@ int fddm_batch_create(fddm_batch** out, const fddm_batch_item* items, int n, int
   if (hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(set_err(err, errlen, FDDM_ERR_HIP, "graph capture could not start"));
   int rc = FDDM_OK;
-  hipEventRecord(b->events[0], origin);
-  for (int j = 0; j < nb; ++j) hipStreamWaitEvent(b->branches[1 + j], b->events[0], 0);
-  for (int i = 0; i < n && !rc; ++i) {
+  hipError_t ge = hipEventRecord(b->events[0], origin);
+  for (int j = 0; j < nb && ge == hipSuccess; ++j) ge = hipStreamWaitEvent(b->branches[1 + j], b->events[0], 0);
+  for (int i = 0; i < n && !rc && ge == hipSuccess; ++i) {
     hipStream_t s = b->branches[1 + (i % nb)];
     rc = fddm_plan_execute(b->plans[i], items[i].d_img, items[i].d_out, b->ws[i], s, err, errlen);
   }
-  for (int j = 0; j < nb; ++j) {
-    hipEventRecord(b->events[1 + j], b->branches[1 + j]);
-    hipStreamWaitEvent(origin, b->events[1 + j], 0);
+  for (int j = 0; j < nb && ge == hipSuccess; ++j) {
+    ge = hipEventRecord(b->events[1 + j], b->branches[1 + j]);
+    if (ge == hipSuccess) ge = hipStreamWaitEvent(origin, b->events[1 + j], 0);
   }
   const hipError_t ce = hipStreamEndCapture(origin, &b->graph);
   if (rc) return fail(rc);
+  if (ge != hipSuccess)
+    return fail(set_err(err, errlen, FDDM_ERR_HIP, "graph capture fork/join failed: %s", hipGetErrorString(ge)));
   if (ce != hipSuccess) return fail(set_err(err, errlen, FDDM_ERR_HIP, "graph capture failed: %s",
                                             hipGetErrorString(ce)));
   if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)

@@ -688,6 +688,7 @@ int fddm_release_cache(char* err, size_t errlen) {
   }
   if (c.stream) (void)hipStreamDestroy(c.stream);
   c = HostPathCache{};
+  azavg_release_scratch(dev);
   return FDDM_OK;
 }
 

@@ -15,5 +15,7 @@ int plan_create_ex(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H,
                    int precision, int out_dtype, int frames_per_block, char* err, size_t errlen);
 // Frames per block chosen for a plan.
 int plan_frames_per_block(const fddm_plan* p);
+// Frees the azimuthal-average scratch kept for device dev.
+void azavg_release_scratch(int dev);
 
 }  // namespace fddm

@@ -222,7 +222,7 @@ struct fddm_server {
   }
 
   void worker(int slot) {
-    hipSetDevice(device);
+    (void)hipSetDevice(device);  // validated by fddm_server_create
     hipStream_t s = streams[slot];
     char err[512];
     for (;;) {
@@ -298,8 +298,8 @@ int fddm_server_create(fddm_server** out, int max_slots, uint64_t hbm_bytes, cha
   s->streams.resize(max_slots);
   for (int i = 0; i < max_slots; ++i) {
     if (hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking) != hipSuccess) {
-      for (int j = 0; j < i; ++j) hipStreamDestroy(s->streams[j]);
-      hipFree(s->base);
+      for (int j = 0; j < i; ++j) (void)hipStreamDestroy(s->streams[j]);
+      (void)hipFree(s->base);
       delete s;
       return set_err(err, errlen, FDDM_ERR_HIP, "stream creation failed");
     }
@@ -323,10 +323,10 @@ int fddm_server_destroy(fddm_server* s) {
   }
   s->cv_work.notify_all();
   for (auto& t : s->workers) t.join();
-  hipSetDevice(s->device);
+  (void)hipSetDevice(s->device);
   for (auto& kv : s->plans) fddm_plan_destroy(kv.second);
-  for (hipStream_t st : s->streams) hipStreamDestroy(st);
-  hipFree(s->base);
+  for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+  (void)hipFree(s->base);
   delete s;
   return FDDM_OK;
 }
@@ -367,7 +367,8 @@ int fddm_server_submit(fddm_server* s, int mode, const void* img, int img_on_dev
   r->img = img;
   r->img_on_device = img_on_device != 0;
   // validates every argument and builds (or finds) the plan
-  hipSetDevice(s->device);
+  if (hipSetDevice(s->device) != hipSuccess)
+    return set_err(err, errlen, FDDM_ERR_DEVICE, "cannot select device %d", s->device);
   int rc = s->plan_for(r.get(), err, errlen);
   if (rc) return rc;
   fddm_plan_workspace_size(r->plan, &r->ws_bytes);
@@ -454,8 +455,7 @@ int fddm_server_fetch(fddm_server* s, uint64_t id, void* host_out, char* err, si
   uint64_t bytes = 0;
   int rc = fddm_server_result(s, id, &d, &bytes, err, errlen);
   if (rc) return rc;
-  hipSetDevice(s->device);
-  if (hipMemcpy(host_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+  if (hipSetDevice(s->device) != hipSuccess || hipMemcpy(host_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
     return set_err(err, errlen, FDDM_ERR_HIP, "result copy failed");
   return FDDM_OK;
 }

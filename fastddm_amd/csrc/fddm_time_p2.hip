@@ -28,6 +28,7 @@
 
 #include "fddm_kernels.hpp"
 #include "fddm_p2.hpp"
+#include "fddm_wave.hpp"
 
 namespace fddm {
 namespace {
@@ -35,24 +36,6 @@ using namespace p2;
 
 constexpr int kMinThreads = 256;  // workgroup size for plans of fewer threads (G q per group)
 constexpr int kEdge = 32;
-
-template <typename A>
-__device__ __forceinline__ A wave_sum(A v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-template <typename A>
-__device__ __forceinline__ A wave_incl_scan(A v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const A w = __shfl_up(v, o, 64);
-    if (lane >= o) v += w;
-  }
-  return v;
-}
 
 // 1 / n for a positive integer n: hardware reciprocal refined by Newton's
 // iteration r <- r (2 - n r), twice for double (error below 2 ulp).
@@ -74,9 +57,9 @@ __device__ __forceinline__ A recip(int n) {
 template <int WPT>
 __device__ __forceinline__ void group_sum2(double& x, double& y, double* red) {
   const int wave = threadIdx.x >> 6;
-  x = wave_sum(x);
-  y = wave_sum(y);
-  if ((threadIdx.x & 63) == 0) {
+  x = wave::incl_scan(x);
+  y = wave::incl_scan(y);
+  if ((threadIdx.x & 63) == 63) {  // the wave totals
     red[2 * wave] = x;
     red[2 * wave + 1] = y;
   }
@@ -259,9 +242,8 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs 
     run += val;
     pre[i] = run;
   }
-  const Acc inc = wave_incl_scan(run);
-  Acc exc = __shfl_up(inc, 1, 64);
-  if (lane == 0) exc = 0;
+  const Acc inc = wave::incl_scan(run);
+  Acc exc = wave::from_prev_lane(inc);  // 0 in lane 0
   if (lane == 63) chunk[wg] = inc;
   __syncthreads();
   Acc ptot = 0;

@@ -87,7 +87,8 @@ int fddm_ddm_diff(const void *img, int dtype, uint64_t length, uint64_t height, 
                   const uint32_t *lags, uint64_t num_lags, uint64_t nx, uint64_t ny,
                   const void *window, int window_f32, int precision, int out_dtype, void *out,
                   char *err, size_t errlen);
-/* Drops the device buffers cached by fddm_ddm_fft / fddm_ddm_diff. */
+/* Drops the device buffers cached by fddm_ddm_fft / fddm_ddm_diff and the
+ * scratch kept by fddm_azimuthal_average, on the current device. */
 int fddm_release_cache(char *err, size_t errlen);
 
 /* ---- plans: device-resident inputs and outputs ----

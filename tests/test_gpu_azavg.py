@@ -69,6 +69,42 @@ def test_device_resident_float32_result(hip):
         b.free()
 
 
+@pytest.mark.parametrize("bins", [300, 600, 1000])
+def test_many_bins(hip, bins):
+    """Wide bin counts take fewer rows per workgroup (8, 4, 2, 1 by the LDS
+    budget); a 128 x 128 result with 1000 bins also leaves bins empty."""
+    img = ar1_speckle(21, 128, 128, sigma=1.5, diffusivity=0.3, mean=900, std=120, seed=8)
+    iso = fddm.ddm(img, np.arange(1, 21), core="hip", mode="fft", precision="fp64")
+    a = fddm.azimuthal_average(iso, bins=bins)
+    avg, err, k, _ = _ref(iso, bins)
+    assert np.allclose(a._data, avg, rtol=1e-12, atol=0, equal_nan=True)
+    assert np.allclose(a._err, err, rtol=1e-10, atol=0, equal_nan=True)
+    assert np.array_equal(np.isnan(a._data), np.isnan(avg))
+
+
+def test_masked_nan_pixels_add_nothing(isf):
+    """NaN values under the mask leave every bin finite, as in the reference,
+    where masked pixels are skipped."""
+    data = isf._data.copy()
+    mask = np.ones(isf.shape[1:], dtype=bool)
+    mask[5:9, 3:12] = False
+    data[:, 5:9, 3:12] = np.nan
+    bad = fddm.ImageStructureFunction(data, isf.kx, isf.ky, isf.width, isf.height, isf.tau)
+    a = fddm.azimuthal_average(bad, bins=10, mask=mask)
+    avg, err, _, _ = _ref(isf, 10, None, mask)
+    assert np.allclose(a._data, avg, rtol=1e-12, atol=0, equal_nan=True)
+    assert np.allclose(a._err, err, rtol=1e-10, atol=0, equal_nan=True)
+    assert np.array_equal(np.isnan(a._data), np.isnan(avg))
+
+
+def test_repeated_calls_reuse_maps(isf):
+    """A second call with the same geometry returns the same values (cached
+    bin maps, reused device scratch)."""
+    a = fddm.azimuthal_average(isf, bins=10)
+    b = fddm.azimuthal_average(isf, bins=10)
+    assert np.array_equal(a._data, b._data, equal_nan=True) and np.array_equal(a._err, b._err, equal_nan=True)
+
+
 def test_rejects_bad_maps(hip):
     d = hip.DeviceBuffer(8 * 3 * 4 * 2)
     w = np.ones(8)
