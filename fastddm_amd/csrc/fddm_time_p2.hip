@@ -75,15 +75,26 @@ __device__ __forceinline__ void group_sum2(double& x, double& y, double* red) {
   y = sy;
 }
 
-// Workgroup shape and LDS layout of the time kernel at N = 2^LOGN.
+// Workgroup shape and LDS layout of the time kernel at N = 2^LOGN.  Plans of
+// fewer than kMinThreads threads put G = kMinThreads / TPT q in a workgroup;
+// compensated plans of 256 or 512 threads put two q in one when two such
+// workgroups still fit a CU's LDS.  The q of one workgroup are neighbours, and the lag
+// loop interleaves them across lanes, so each lag store covers G adjacent
+// output values instead of one.
 template <int LOGN, typename T, bool SUB>
 struct P2Lds {
   static constexpr P2Plan P = p2_plan(LOGN, sizeof(T) == 8);
-  static constexpr int N = 1 << LOGN, TPT = P.threads;
-  static constexpr int NT = TPT > kMinThreads ? TPT : kMinThreads;  // workgroup size
-  static constexpr int G = NT / TPT, WPT = TPT / 64;
+  static constexpr int N = 1 << LOGN, TPT = P.threads, WPT = TPT / 64;
   using Acc = std::conditional_t<SUB, double, float>;
   static constexpr size_t csz = sizeof(cplx_t<T>);
+  static constexpr size_t group_bytes =
+      lds_padded_size(N) * csz + 2 * kEdge * csz + 2 * WPT * sizeof(double) + WPT * sizeof(Acc);
+  // (the uncompensated FP32 kernel would spill at 8 waves per SIMD: not paired)
+  static constexpr bool kPair = SUB && TPT >= 256 && TPT <= 512 && 4 * group_bytes <= 160 * 1024;
+  static constexpr int NT = TPT < kMinThreads ? kMinThreads : (kPair ? 2 * TPT : TPT);  // workgroup size
+  static constexpr int G = NT / TPT;
+  // paired workgroups: two per CU hold 8 waves per SIMD when at most 64 VGPRs are used
+  static constexpr int kMinWaves = kPair ? 8 : 1;
   static constexpr size_t buf = static_cast<size_t>(G) * lds_padded_size(N) * csz;
   static constexpr size_t edge = static_cast<size_t>(G) * 2 * kEdge * csz;
   static constexpr size_t red = 2 * (NT / 64) * sizeof(double);
@@ -114,7 +125,8 @@ __device__ __forceinline__ void p2_passes(cplx_t<T>* buf, const cplx_t<T>* __res
 }
 
 template <typename T, typename OutT, int LOGN, bool SUB>
-__global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs a, int wg_per_xcd, int nq) {
+__global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB>::kMinWaves))
+    k_time_p2(TimeArgs a, int wg_per_xcd, int nq) {
   using C = cplx_t<T>;
   using Lay = P2Lds<LOGN, T, SUB>;
   using Acc = typename Lay::Acc;
@@ -131,14 +143,18 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs 
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
   const int g = tid / TPT, u = tid - g * TPT, lane = tid & 63, wg = u >> 6;
-  C* const buf = reinterpret_cast<C*>(smem) + g * lds_padded_size(N);
-  C* const edge = reinterpret_cast<C*>(smem + Lay::buf) + g * 2 * kEdge;
+  C* const buf0 = reinterpret_cast<C*>(smem);
+  C* const edge0 = reinterpret_cast<C*>(smem + Lay::buf);
   double* const red = reinterpret_cast<double*>(smem + Lay::buf + Lay::edge);
-  Acc* const chunk = reinterpret_cast<Acc*>(smem + Lay::buf + Lay::edge + Lay::red) + g * WPT;
+  Acc* const chunk0 = reinterpret_cast<Acc*>(smem + Lay::buf + Lay::edge + Lay::red);
+  C* const buf = buf0 + g * lds_padded_size(N);
+  C* const edge = edge0 + g * 2 * kEdge;
+  Acc* const chunk = chunk0 + g * WPT;
 
   // Workgroups are dealt round-robin to the 8 XCDs: each XCD gets a
   // contiguous range of q so neighbouring output columns share its L2.
-  const int q = ((blockIdx.x & 7) * wg_per_xcd + (blockIdx.x >> 3)) * G + g;
+  const int qbase = ((blockIdx.x & 7) * wg_per_xcd + (blockIdx.x >> 3)) * G;
+  const int q = qbase + g;
   const bool active = q < nq;
   const int L = a.L;
   const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(active ? q : 0) * a.Lp;
@@ -259,43 +275,57 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT)) k_time_p2(TimeArgs 
     if (i < seg && t < L - 1) tail[t] = pre[i] + exc;
   }
   __syncthreads();
-  if (!active) return;
 
+  // 6. lags: thread tid serves q group h = tid % G and lag indices
+  // nl - 1 - tid / G, stepping by NT / G, so neighbouring lanes store the same
+  // lag of neighbouring q.  Lags in reverse order: the largest lags (direct
+  // sums, longest loops) all fall in the first wave(s).
   OutT* __restrict__ out = static_cast<OutT*>(a.out);
   const size_t nqs = static_cast<size_t>(a.ny) * a.nxh;
-  const int ky = q / a.nxh, kx = q - ky * a.nxh;
-  const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
   const Acc inv_n = static_cast<Acc>(1) / static_cast<Acc>(N);
-  // lags in reverse order: the largest lags (direct sums, longest loops) all
-  // fall in the first wave of the group
-  for (int i = a.nl - 1 - u; i >= 0; i -= TPT) {
-    const int d = static_cast<int>(a.lags[i]);
-    Acc D;
-    if (SUB && L - d <= kEdge) {
-      Acc acc = 0;
-      for (int t = 0; t < L - d; ++t) {
-        const int s = t + d;
-        const C x = s < kEdge ? edge[s] : edge[s - L + 2 * kEdge];
-        const C y = edge[t];
-        const Acc dr = static_cast<Acc>(x.x) - static_cast<Acc>(y.x);
-        const Acc di = static_cast<Acc>(x.y) - static_cast<Acc>(y.y);
-        acc += dr * dr + di * di;
+  {
+    const int h = tid % G, v = tid / G, qh = qbase + h;
+    if (qh < nq) {
+      const C* const hbuf = buf0 + h * lds_padded_size(N);
+      const Acc* const htail = reinterpret_cast<const Acc*>(hbuf + padi(NH));
+      const C* const hedge = edge0 + h * 2 * kEdge;
+      Acc hptot = 0;
+#pragma unroll
+      for (int w = 0; w < WPT; ++w) hptot += chunk0[h * WPT + w];
+      const int ky = qh / a.nxh, kx = qh - ky * a.nxh;
+      const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
+      for (int i = a.nl - 1 - v; i >= 0; i -= Lay::NT / G) {
+        const int d = static_cast<int>(a.lags[i]);
+        Acc D;
+        if (SUB && L - d <= kEdge) {
+          Acc acc = 0;
+          for (int t = 0; t < L - d; ++t) {
+            const int s = t + d;
+            const C x = s < kEdge ? hedge[s] : hedge[s - L + 2 * kEdge];
+            const C y = hedge[t];
+            const Acc dr = static_cast<Acc>(x.x) - static_cast<Acc>(y.x);
+            const Acc di = static_cast<Acc>(x.y) - static_cast<Acc>(y.y);
+            acc += dr * dr + di * di;
+          }
+          D = acc * recip<Acc>(L - d);
+        } else {
+          const Acc head = htail[L - d - 1];   // psum(L - d)
+          const Acc tl = hptot - htail[d - 1];  // psum(L) - psum(d)
+          // N C(d) = Re FFT_N(P)(d) from A = Z(d), B = Z(N/2 - d) and
+          // (c, s) = (cos, sin)(2 pi d / N): 2 Re X = A.x + B.x + c (A.y + B.y) - s (A.x - B.x)
+          const C za = hbuf[padi(d)], zb = hbuf[padi((NH - d) & (NH - 1))], cs = tw[TWP + d];
+          const Acc ax = za.x, ay = za.y, bx = zb.x, by = zb.y;
+          const Acc re = (ax + bx) + static_cast<Acc>(cs.x) * (ay + by) - static_cast<Acc>(cs.y) * (ax - bx);
+          const Acc c = re * (Acc(0.5) * inv_n);
+          D = (head + tl - 2 * c) * recip<Acc>(L - d);
+        }
+        out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(D);
       }
-      D = acc * recip<Acc>(L - d);
-    } else {
-      const Acc head = tail[L - d - 1];   // psum(L - d)
-      const Acc tl = ptot - tail[d - 1];  // psum(L) - psum(d)
-      // N C(d) = Re FFT_N(P)(d) from A = Z(d), B = Z(N/2 - d) and
-      // (c, s) = (cos, sin)(2 pi d / N): 2 Re X = A.x + B.x + c (A.y + B.y) - s (A.x - B.x)
-      const C za = buf[padi(d)], zb = buf[padi((NH - d) & (NH - 1))], cs = tw[TWP + d];
-      const Acc ax = za.x, ay = za.y, bx = zb.x, by = zb.y;
-      const Acc re = (ax + bx) + static_cast<Acc>(cs.x) * (ay + by) - static_cast<Acc>(cs.y) * (ax - bx);
-      const Acc c = re * (Acc(0.5) * inv_n);
-      D = (head + tl - 2 * c) * recip<Acc>(L - d);
     }
-    out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(D);
   }
-  if (u == 0) {
+  if (active && u == 0) {
+    const int ky = q / a.nxh, kx = q - ky * a.nxh;
+    const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
     const Acc mr2 = static_cast<Acc>(fsr / L), mi2 = static_cast<Acc>(fsi / L);
     Acc ps, var;
     if constexpr (SUB) {
