@@ -11,6 +11,8 @@
 //              S[ky * nxh + kx][t] straight from registers.  Slabs above
 //              64 KiB leave room for two workgroups per CU, which then run
 //              1024 threads each to keep 32 waves resident.
+#include <algorithm>
+
 #include "fddm_kernels.hpp"
 #include "fddm_p2.hpp"
 
@@ -112,6 +114,91 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a) {
   }
 }
 
+// Full-width rows of 2-byte pixels (W == NX <= 512), FP32: a persistent form
+// of k_rows_p2.  Workgroup b takes items b, b + grid, ... (item = tb * H + y);
+// the next item's pixels and frame-0 row are loaded into registers before
+// the current item's FFT and stores, so their latency overlaps the transform
+// instead of opening each item.
+template <typename Tin, typename T, int LOGX, int NP>
+__global__ void __launch_bounds__(kThreads, 8) k_rows_p2_pf(RowArgs a, uint32_t nitems) {
+  using C = cplx_t<T>;
+  constexpr int NX = 1 << LOGX, NXH = NX / 2 + 1, TB = 2 * NP;
+  constexpr SpPlan P = sp_plan(LOGX);
+  constexpr int VX = 8 / sizeof(Tin), GV = NX / VX;
+  constexpr int PER = (NP * GV + kThreads - 1) / kThreads;  // 8-byte vectors per thread and frame row
+  struct alignas(8) Vec {
+    Tin v[VX];
+  };
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* const buf = reinterpret_cast<C*>(smem);
+  const int tid = threadIdx.x;
+  const int H = a.H;
+  const Tin* __restrict__ img = static_cast<const Tin*>(a.img);
+  const T* __restrict__ win = static_cast<const T*>(a.window);
+  const size_t plane = static_cast<size_t>(H) * NX;
+  const bool sub = a.subtract0 != 0;
+  const C* __restrict__ tw = static_cast<const C*>(a.twx);
+  const T half_norm = static_cast<T>(0.5 * a.norm);
+  const uint32_t kstride = static_cast<uint32_t>(H) * TB;
+
+  Vec va[PER], vb[PER], v0[PER];
+  auto load = [&](uint32_t item) {
+    const int y = static_cast<int>(item % H);
+    const uint64_t t0 = static_cast<uint64_t>(item / H) * TB;
+    const Tin* __restrict__ row0 = img + static_cast<size_t>(y) * NX;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + k * kThreads;
+      if ((NP * GV) % kThreads != 0 && e >= NP * GV) break;
+      const int p = e / GV, x0 = (e - p * GV) * VX;
+      const uint64_t ta = t0 + 2 * p;
+      va[k] = ta < a.frames ? *reinterpret_cast<const Vec*>(row0 + ta * plane + x0) : Vec{};
+      vb[k] = ta + 1 < a.frames ? *reinterpret_cast<const Vec*>(row0 + (ta + 1) * plane + x0) : Vec{};
+      v0[k] = sub ? *reinterpret_cast<const Vec*>(row0 + x0) : Vec{};
+    }
+  };
+
+  uint32_t item = blockIdx.x;
+  if (item < nitems) load(item);
+  for (; item < nitems; item += gridDim.x) {
+    const int y = static_cast<int>(item % H);
+    const uint64_t tb = item / H, t0 = tb * TB;
+    // registers -> LDS: pairs (2p, 2p+1) as one complex row, element (p, x)
+    // at logical p NX + x
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + k * kThreads;
+      if ((NP * GV) % kThreads != 0 && e >= NP * GV) break;
+      const int p = e / GV, x0 = (e - p * GV) * VX;
+      const uint64_t ta = t0 + 2 * p;
+      C* d = buf + padi(p * NX + x0);  // VX divides 32: the VX slots are contiguous
+#pragma unroll
+      for (int v = 0; v < VX; ++v) {
+        const T w = win ? win[static_cast<size_t>(y) * NX + x0 + v] : T(1);
+        const T fa = ta < a.frames ? pixel_value<Tin, T>(va[k].v[v], v0[k].v[v], sub, w) : T(0);
+        const T fb = ta + 1 < a.frames ? pixel_value<Tin, T>(vb[k].v[v], v0[k].v[v], sub, w) : T(0);
+        d[v] = C{fa, fb};
+      }
+    }
+    __syncthreads();
+    if (item + gridDim.x < nitems) load(item + gridDim.x);
+    fft3_lds<T, NX, P.r0, P.r1, P.r2, kThreads, NP, false, 1, NX>(buf, tw, tid);
+
+    // split into the two half spectra and store R[tb][k][y][ti] (ti fastest)
+    C* __restrict__ R = static_cast<C*>(a.R) + (tb * NXH * static_cast<size_t>(H) + y) * TB;
+    for (int w = tid; w < NXH * TB; w += kThreads) {
+      const int ti = w & (TB - 1), k = w / TB;
+      const int p = ti >> 1;
+      const C zk = buf[padi(p * NX + k)];
+      const C zn = buf[padi(p * NX + ((NX - k) & (NX - 1)))];
+      const C o = (ti & 1) == 0 ? C{(zk.x + zn.x) * half_norm, (zk.y - zn.y) * half_norm}
+                                : C{(zk.y + zn.y) * half_norm, (zn.x - zk.x) * half_norm};
+      R[static_cast<uint32_t>(k) * kstride + ti] = o;
+    }
+    __syncthreads();  // the LDS rows are rewritten by the next item
+  }
+}
+
 template <typename T, int LOGY, int TB, int NT>
 __global__ void __launch_bounds__(NT) k_cols_p2(ColArgs a) {
   using C = cplx_t<T>;
@@ -166,6 +253,28 @@ __global__ void __launch_bounds__(NT) k_cols_p2(ColArgs a) {
 template <int LOGX, typename Tin, typename T, int NP>
 hipError_t rows_np(const RowArgs& a, unsigned grid, hipStream_t s) {
   const size_t lds = sizeof(cplx_t<T>) * lds_padded_size(NP * (1 << LOGX));
+  // persistent form (FP32, 2-byte pixels, rows up to 512 wide: the cases whose
+  // prefetch registers fit 8 waves per SIMD without spills); as many
+  // workgroups as stay resident
+  if constexpr (sizeof(Tin) == 2 && sizeof(T) == 4 && LOGX <= 9) {
+    if (a.W == (1 << LOGX)) {
+      auto kernel = k_rows_p2_pf<Tin, T, LOGX, NP>;
+      if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+      }
+      int dev = 0, cus = 0, per_cu = 0;
+      hipError_t e = hipGetDevice(&dev);
+      if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), kThreads, lds);
+      if (e != hipSuccess) return e;
+      const unsigned resident = static_cast<unsigned>(std::max(1, cus) * std::max(1, per_cu));
+      hipLaunchKernelGGL(kernel, dim3(std::min(grid, resident)), dim3(kThreads), lds, s, a, static_cast<uint32_t>(grid));
+      return hipGetLastError();
+    }
+  }
   auto kernel = k_rows_p2<Tin, T, LOGX, NP>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),

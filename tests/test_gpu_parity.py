@@ -98,6 +98,33 @@ def test_spatial_spectra_match_oracle(hip, precision, shape, nx, ny, window):
     assert err < (1e-13 if precision == "fp64" else 2e-6), err
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+@pytest.mark.parametrize("shape,dtype,window", [
+    ((131, 256, 256), np.uint16, True),   # 9 frame blocks x 256 rows; odd L: last pair half empty
+    ((70, 512, 512), np.int16, False),    # 5 frame blocks x 512 rows, signed pixels
+])
+def test_persistent_rows_match_oracle(hip, precision, shape, dtype, window):
+    """More (row, frame block) items than resident workgroups, so each
+    workgroup of the persistent row kernel loops and prefetches."""
+    L, H, W = shape
+    info = np.iinfo(dtype)
+    img = np.random.default_rng(13).integers(max(info.min, -3000), 4000, shape).astype(dtype)
+    win = np.random.default_rng(14).random((H, W)) if window else None
+    plan = hip.Plan("fft", img.dtype, shape, [1], W, H, next_fast_len(2 * L, "hip"), win, precision)
+    d_img = hip.DeviceBuffer.from_array(img)
+    d_ws = hip.DeviceBuffer(plan.workspace_size())
+    got = plan.spectra(d_img, d_ws)
+    base = img.astype(np.float64)
+    if precision != "fp32":
+        base = base - base[0]
+    ref = onp.spectra(base, W, H, win).reshape(L, -1).T
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err < 2e-6, err
+    plan.destroy()
+    d_img.free()
+    d_ws.free()
+
+
 # ------------------------------------------------ whole path against the oracle
 CASES = [
     # (L, H, W, nx, ny, window, lags)
