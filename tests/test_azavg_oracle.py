@@ -93,3 +93,35 @@ def test_bin_maps_edge_rules():
         bin_maps(dist, 3, weights=-np.ones_like(dist))
     with pytest.warns(UserWarning):
         bin_maps(dist, 3, mask=np.ones_like(dist))
+
+
+def test_device_maps_cache():
+    """Bin maps are kept per geometry: the same inputs return the same
+    (read-only) arrays, a different bin count or mask builds new ones, and a
+    non-boolean mask is never cached (its cast warns on every call)."""
+    from fastddm_amd import azimuthalaverage as az
+
+    az._MAPS_CACHE.clear()
+    _, kx, ky = _grid(16, 14)
+    a = az._device_maps(kx, ky, 14, 16, 8, 10, None, None, None, True)
+    b = az._device_maps(kx.copy(), ky.copy(), 14, 16, 8, 10, None, None, None, True)
+    assert all(x is y for x, y in zip(a, b))
+    assert not a[0].flags.writeable and not a[1].flags.writeable
+    c = az._device_maps(kx, ky, 14, 16, 8, 12, None, None, None, True)
+    assert len(c[3]) == 12 and c[0] is not a[0]
+    mask = np.ones((16, 8), dtype=bool)
+    mask[3, 2] = False
+    d = az._device_maps(kx, ky, 14, 16, 8, 10, None, mask, None, True)
+    assert d[1][3, 2] == 0 and a[1][3, 2] != 0
+    with pytest.warns(UserWarning):
+        az._device_maps(kx, ky, 14, 16, 8, 10, None, mask.astype(int), None, True)
+    with pytest.warns(UserWarning):
+        az._device_maps(kx, ky, 14, 16, 8, 10, None, mask.astype(int), None, True)
+    for i in range(2 * az._MAPS_CACHE_SIZE):
+        az._device_maps(kx, ky, 14, 16, 8, 20 + i, None, None, None, True)
+    assert len(az._MAPS_CACHE) == az._MAPS_CACHE_SIZE
+    # the maps equal a fresh build
+    dist = np.sqrt(np.meshgrid(kx, ky)[0] ** 2 + np.meshgrid(kx, ky)[1] ** 2)
+    fresh = bin_maps(dist, 10, None, None, None, az._half_plane_counts((16, 8), 14), True)
+    for x, y in zip(a, fresh):
+        assert np.array_equal(x, y)
