@@ -6,7 +6,7 @@
 //   k_cols  stage (a) column pass + stage (b) transpose: length-ny FFT of Tb
 //           frames of one kx column at once, store q-major S[q][t] in runs of
 //           Tb consecutive frames.
-//   k_diff  stage (d) + (e): per-q direct lag differences, PS and VAR.
+//   k_diff  stage (d) + (e): direct lag differences, one wave per q, PS and VAR.
 //
 // The fft-mode time stage lives in fddm_time_p2.hip (power-of-two lengths).
 //
@@ -17,6 +17,7 @@
 
 #include "fddm_fft.hpp"
 #include "fddm_kernels.hpp"
+#include "fddm_wave.hpp"
 
 namespace fddm {
 
@@ -31,29 +32,6 @@ int fft_bucket(long long work) {
   if (e <= 32) return 32;
   if (e <= 64) return 64;
   return 0;
-}
-
-// ---------------------------------------------------------------------------
-// Wave / block reductions (wave64).
-// ---------------------------------------------------------------------------
-template <typename A>
-__device__ __forceinline__ A wave_sum(A v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-template <typename A>
-__device__ __forceinline__ A block_sum(A v) {
-  __shared__ A sh[kWaves];
-  const A w = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = w;
-  __syncthreads();
-  A tot = 0;
-#pragma unroll
-  for (int i = 0; i < kWaves; ++i) tot += sh[i];
-  __syncthreads();
-  return tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -179,53 +157,111 @@ __device__ __forceinline__ size_t shifted_q(int q, int ny, int nxh) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage (d)+(e): direct differences per q.
+// Stage (d)+(e): direct differences.  A wave works on QPW q at once, one per
+// segment of SEG = 64 / QPW lanes (QPW = 4: one q per 16-lane DPP row, 2: one
+// per half wave); the
+// segment's series sits in its own LDS slice when the workgroup's slices fit
+// 64 KiB (else it is read from L2).  Lane l of a segment takes t = l, l + SEG,
+// ...; lags go in batches of kLagBatch, so each F(t) load serves kLagBatch
+// differences, and each lag's SEG partial sums are added by a DPP scan inside
+// the segment, whose last lane writes D: one 4-step scan serves four q, one
+// 5-step scan two.  Differences are formed after widening to Acc, so they are exact on
+// the compensated paths.
 // ---------------------------------------------------------------------------
-template <typename T, typename OutT, int PREC, bool USE_LDS>
-__global__ void __launch_bounds__(kThreads) k_diff(TimeArgs a, int q_per_xcd, int nq) {
+constexpr int kLagBatch = 4;
+
+// Inclusive scan inside segments of SEG lanes (16: DPP rows; 32: wave
+// halves, rows 1 and 3 adding the last lane of rows 0 and 2; 64: the wave).
+template <int SEG, typename A>
+__device__ __forceinline__ A seg_total_scan(A v) {
+  if constexpr (SEG == 64) {
+    return wave::incl_scan(v);
+  } else {
+    static_assert(SEG == 16 || SEG == 32, "segment");
+    v += wave::dpp_zero<wave::kRowShr1, 0xf>(v);
+    v += wave::dpp_zero<wave::kRowShr2, 0xf>(v);
+    v += wave::dpp_zero<wave::kRowShr4, 0xf>(v);
+    v += wave::dpp_zero<wave::kRowShr8, 0xf>(v);
+    if constexpr (SEG == 32) v += wave::dpp_zero<wave::kRowBcast15, 0xa>(v);
+    return v;
+  }
+}
+
+template <typename T, typename OutT, int PREC, bool USE_LDS, int QPW>
+__global__ void __launch_bounds__(kThreads) k_diff(TimeArgs a, int wg_per_xcd, int nq) {
   using C = cplx_t<T>;
   using Acc = std::conditional_t<PREC == PREC_FP32, float, double>;
+  constexpr int SEG = 64 / QPW;
   extern __shared__ __align__(16) unsigned char smem[];
-  C* buf = reinterpret_cast<C*>(smem);
-  const int tid = threadIdx.x;
-  const int q = (blockIdx.x & 7) * q_per_xcd + (blockIdx.x >> 3);
-  if (q >= nq) return;
-  const int L = a.L;
-  const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(q) * a.Lp;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int sl = lane & (SEG - 1), seg = lane / SEG;
+  const int qw = ((blockIdx.x & 7) * wg_per_xcd + (blockIdx.x >> 3)) * kWaves * QPW + wid * QPW;
+  if (qw >= nq) return;  // whole waves: nothing below synchronises the workgroup
+  const int q = qw + seg;
+  const bool valid = q < nq;
+  const int L = a.L, Lv = valid ? L : 0;  // segments past the last q run empty loops
+  const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(valid ? q : qw) * a.Lp;
   const C* ser = src;
   if constexpr (USE_LDS) {
-    for (int t = tid; t < L; t += kThreads) buf[t] = src[t];
-    __syncthreads();
-    ser = buf;
+    C* slice = reinterpret_cast<C*>(smem) + static_cast<size_t>(wid * QPW + seg) * L;
+    for (int t0 = sl; t0 < Lv; t0 += 4 * SEG) {  // four loads in flight per lane
+      C v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[k] = t0 + SEG * k < Lv ? src[t0 + SEG * k] : C{0, 0};
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (t0 + SEG * k < Lv) slice[t0 + SEG * k] = v[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    ser = slice;
   }
   Acc sr = 0, si = 0, s2 = 0;
-  for (int t = tid; t < L; t += kThreads) {
+  for (int t = sl; t < Lv; t += SEG) {
     const C f = ser[t];
-    sr += static_cast<Acc>(f.x);
-    si += static_cast<Acc>(f.y);
-    s2 += static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
+    const Acc fx = static_cast<Acc>(f.x), fy = static_cast<Acc>(f.y);
+    sr += fx;
+    si += fy;
+    s2 += fx * fx + fy * fy;
   }
-  sr = block_sum(sr);
-  si = block_sum(si);
-  s2 = block_sum(s2);
+  sr = seg_total_scan<SEG>(sr);  // the segment's last lane holds the sums
+  si = seg_total_scan<SEG>(si);
+  s2 = seg_total_scan<SEG>(s2);
+  const bool writer = valid && sl == SEG - 1;
 
   OutT* __restrict__ out = static_cast<OutT*>(a.out);
   const size_t nqs = static_cast<size_t>(a.ny) * a.nxh;
-  const size_t oq = shifted_q(q, a.ny, a.nxh);
-  const int lane = tid & 63, wid = tid >> 6;
-  for (int i = wid; i < a.nl; i += kWaves) {
-    const int d = static_cast<int>(a.lags[i]);
-    Acc acc = 0;
-    for (int t = lane; t + d < L; t += 64) {
-      const C u = ser[t + d], v = ser[t];
-      const Acc dr = static_cast<Acc>(u.x) - static_cast<Acc>(v.x);
-      const Acc di = static_cast<Acc>(u.y) - static_cast<Acc>(v.y);
-      acc += dr * dr + di * di;
+  const size_t oq = shifted_q(valid ? q : qw, a.ny, a.nxh);
+  for (int i0 = 0; i0 < a.nl; i0 += kLagBatch) {
+    int d[kLagBatch];
+#pragma unroll
+    for (int k = 0; k < kLagBatch; ++k) d[k] = i0 + k < a.nl ? static_cast<int>(a.lags[i0 + k]) : L;
+    int dmin = d[0];
+#pragma unroll
+    for (int k = 1; k < kLagBatch; ++k) dmin = d[k] < dmin ? d[k] : dmin;
+    Acc acc[kLagBatch] = {};
+    for (int t = sl; t + dmin < Lv; t += SEG) {
+      const C v = ser[t];
+      const Acc vx = static_cast<Acc>(v.x), vy = static_cast<Acc>(v.y);
+#pragma unroll
+      for (int k = 0; k < kLagBatch; ++k) {
+        if (t + d[k] < L) {
+          const C u = ser[t + d[k]];
+          const Acc dr = static_cast<Acc>(u.x) - vx;
+          const Acc di = static_cast<Acc>(u.y) - vy;
+          acc[k] += dr * dr + di * di;
+        }
+      }
     }
-    acc = wave_sum(acc);
-    if (lane == 0) out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(acc / static_cast<Acc>(L - d));
+#pragma unroll
+    for (int k = 0; k < kLagBatch; ++k) {
+      const Acc tot = seg_total_scan<SEG>(acc[k]);
+      if (writer && i0 + k < a.nl)
+        out[static_cast<size_t>(i0 + k) * nqs + oq] = static_cast<OutT>(tot / static_cast<Acc>(L - d[k]));
+    }
   }
-  if (tid == 0) {
+  if (writer) {
     const Acc mr = sr / L, mi = si / L;
     Acc ps, var;
     if (a.S0) {
@@ -333,14 +369,23 @@ hipError_t launch_cols(const ColArgs& a, int precision, hipStream_t s) {
 }
 
 namespace {
+// The most q per wave (4, 2, 1) whose staged series leave room for 8
+// workgroups (32 waves) per CU; else one q per wave, LDS-staged while the
+// workgroup's four series fit 64 KiB, else read from L2.
 template <typename T, typename OutT, int PREC>
 hipError_t diff_l(const TimeArgs& a, hipStream_t s) {
   const int nq = a.ny * a.nxh;
-  const int qpx = (nq + 7) / 8;
-  const unsigned grid = static_cast<unsigned>(qpx) * 8u;
-  const size_t lds = sizeof(cplx_t<T>) * a.L;
-  if (lds <= 64 * 1024) return launch(k_diff<T, OutT, PREC, true>, grid, lds, s, a, qpx, nq);
-  return launch(k_diff<T, OutT, PREC, false>, grid, 0, s, a, qpx, nq);
+  const size_t series = sizeof(cplx_t<T>) * a.L;
+  constexpr size_t kFull = 160 * 1024 / 8;  // LDS per workgroup at 8 workgroups per CU
+  auto go = [&](auto kernel, int qpw, size_t lds) {
+    const int wgs = (nq + kWaves * qpw - 1) / (kWaves * qpw);
+    const int wpx = (wgs + 7) / 8;
+    return launch(kernel, static_cast<unsigned>(wpx) * 8u, lds, s, a, wpx, nq);
+  };
+  if (series * kWaves * 4 <= kFull) return go(k_diff<T, OutT, PREC, true, 4>, 4, series * kWaves * 4);
+  if (series * kWaves * 2 <= kFull) return go(k_diff<T, OutT, PREC, true, 2>, 2, series * kWaves * 2);
+  if (series * kWaves <= 64 * 1024) return go(k_diff<T, OutT, PREC, true, 1>, 1, series * kWaves);
+  return go(k_diff<T, OutT, PREC, false, 1>, 1, 0);
 }
 }  // namespace
 

@@ -179,10 +179,11 @@ def test_every_time_plan_matches_oracle(hip, L, N, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32c"])
-@pytest.mark.parametrize("L", [1500, 5000])
+@pytest.mark.parametrize("L", [120, 200, 1500, 5000])
 def test_diff_mode_long_series(hip, L, precision):
-    """Series longer than the LDS-staged diff kernel takes (fp64 above 4096
-    frames, both precisions at 5000 on the direct-read path)."""
+    """Every shape of the diff kernel: four q per wave (fp32c at 120), two
+    (fp64 at 120, fp32c at 200), one q per wave from LDS (fp64 at 200, fp32c
+    at 1500) and from L2 (fp64 at 1500, both at 5000)."""
     img = ar1_speckle(L, 8, 9, sigma=1.0, diffusivity=0.1, mean=600, std=90, seed=L + 1)
     lags = np.unique(np.geomspace(1, L - 1, 50).astype(int))
     ref = onp.ddm_fft(img, lags, 9, 8, 2 * L, None, subtract_mean=True)
