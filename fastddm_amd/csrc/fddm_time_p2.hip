@@ -137,6 +137,7 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
   using A0 = Pass<T, N, R0, 1, TPT>;                          // first pass (registers)
   using AL = Pass<T, N, RL, N / RL, TPT>;                     // last pass (to registers)
   static_assert(N % (R0 * TPT) == 0, "whole first-pass butterflies per thread");
+  static_assert((N / RL) % TPT == 0, "whole last-pass butterflies per thread (|G|^2 is formed from registers)");
   constexpr int TWA = p2_tw_a(LOGN, F64, NP - 1), TWP = p2_tw_post(LOGN, F64);
   constexpr int MB0 = A0::MB, M0 = A0::M, RH = R0 / 2;
 
