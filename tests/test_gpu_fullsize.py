@@ -1,0 +1,60 @@
+"""The benchmark's workload at its full size -- 2000 frames of 512 x 512
+uint16, lags 1..1999, Blackman-Harris window, nt = 4000 -- checked through
+properties that hold at any size, since the oracle cannot run there in
+seconds:
+
+* the compensated FP32 path agrees with the FP64 path to a median relative
+  error of 1e-5 (and a bounded worst case);
+* D is a mean of squared moduli: positive for these noisy frames, and
+  PS >= VAR >= 0;
+* direct differences (diff mode) on a subset of lags reproduce the
+  Wiener-Khinchin (fft mode) values in FP64.
+"""
+
+import numpy as np
+import pytest
+
+from fastddm_amd import window as fwin
+
+pytestmark = pytest.mark.gpu
+
+L, H, W, NT = 2000, 512, 512, 4000
+
+
+def _run(hip, d_img, mode, lags, precision):
+    win = fwin.blackman_harris((H, W))
+    plan = hip.Plan(mode, np.uint16, (L, H, W), lags, W, H, NT if mode == "fft" else 0, win, precision)
+    d_out = hip.DeviceBuffer(plan.output_size())
+    d_ws = hip.DeviceBuffer(plan.workspace_size())
+    plan.execute(d_img, d_out, d_ws)
+    out = d_out.to_array(plan.out_shape, np.float64)
+    plan.destroy()
+    d_out.free()
+    d_ws.free()
+    return out
+
+
+def test_full_size_properties(hip):
+    print(f"full size: {L} x {H} x {W} uint16, all lags", flush=True)
+    d_img = hip.DeviceBuffer(L * H * W * 2)
+    hip.fill_uniform(d_img, L * H * W, np.uint16, 11, 800, 1200)
+    lags = np.arange(1, L)
+    ref = _run(hip, d_img, "fft", lags, "fp64")
+    got = _run(hip, d_img, "fft", lags, "fp32c")
+    assert ref.shape == got.shape == (L + 1, H, W // 2 + 1)
+
+    nz = ref != 0
+    rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
+    assert np.median(rel) <= 1e-5, np.median(rel)
+    assert rel.max() <= 1e-2, rel.max()
+    del got, rel, nz
+
+    D, PS, VAR = ref[:-2], ref[-2], ref[-1]
+    assert np.all(D > 0)
+    assert np.all(VAR >= 0) and np.all(PS >= VAR)
+
+    sub = np.array([1, 2, 17, 640, 1500, 1999])
+    direct = _run(hip, d_img, "diff", sub, "fp64")
+    assert np.allclose(direct[:-2], D[sub - 1], rtol=1e-9, atol=0)
+    assert np.allclose(direct[-2:], ref[-2:], rtol=1e-9, atol=0)
+    d_img.free()
