@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_VECTOR_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 
 
 def log(msg):
@@ -531,8 +532,9 @@ def main():
         "host_to_host": {
             "frames_per_s": L / e2e_s,
             "ms_per_call": e2e_s * 1e3,
-            "note": "ddm_fft through the C ABI from a host numpy stack to a host numpy result "
-                    "(H2D of the uint16 frames and D2H of the float64 output included)",
+            "note": (f"{'ddm_diff' if diff else 'ddm_fft'} through the C ABI from a host numpy stack to a "
+                     f"host numpy result (H2D of the {img.dtype} frames and D2H of the float64 output "
+                     "included)"),
         },
         "whole_path": {
             "algorithmic_bytes": b["whole"],
@@ -540,6 +542,13 @@ def main():
             "frac_of_hbm_peak": b["whole"] / (ms32 / 1e3) / 1e9 / HBM_PEAK_GBS,
         },
     }
+    if diff:
+        # the diff kernel is compute bound: its FP64 rate beside the HBM line
+        # (6 flops per difference: 2 subtractions, 2 squares, 2 additions)
+        pairs = nq * int(np.sum(L - np.asarray(lags)))
+        tflops = 6.0 * pairs / (st32["time_or_diff"] / 1e3) / 1e12
+        result["roofline"]["valu"] = {"achieved": tflops, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                      "frac": tflops / FP64_VECTOR_PEAK_TFLOPS, "differences": pairs}
     if serve is not None:
         result["serve"] = serve
     if fp64 is not None:
