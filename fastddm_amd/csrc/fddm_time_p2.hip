@@ -15,14 +15,15 @@
 //   2. compensated paths: subtracts the temporal mean (one group reduction);
 //   3. FFT1 (radices R0 .. RL); the last pass stays in registers, |G|^2 is
 //      formed there and written to LDS as z(n) = P(2n) + i P(2n+1);
-//   4. the second forward transform of the real P runs as one complex
-//      transform of length N/2 of z (p2_half_plan), left in LDS; for each lag
-//      Re FFT_N(P)(d) = N C(d) is untangled from Z(d) and Z(N/2 - d);
-//   5. re-reads the series (cache resident) for the
-//      prefix sums of |F(t)|^2 (double on the compensated paths; contiguous
-//      segments per thread and one group scan) and keeps the first and last
-//      kEdge samples; then evaluates D for every requested lag (direct sums
-//      when L - dt <= kEdge), PS and VAR, and stores them fftshifted along y.
+//   4. re-reads the series (cache resident) for the prefix sums of |F(t)|^2
+//      (double on the compensated paths; contiguous segments per thread and
+//      one group scan) into the buffer's second half, and keeps the first and
+//      last kEdge samples;
+//   5. the second forward transform of the real P runs as one complex
+//      transform of length N/2 of z (p2_half_plan) in the first half; for
+//      each lag Re FFT_N(P)(d) = N C(d) is untangled from Z(d) and Z(N/2 - d);
+//   6. evaluates D for every requested lag (direct sums when L - dt <= kEdge),
+//      PS and VAR, and stores them fftshifted along y.
 #include <type_traits>
 #include <utility>
 
@@ -226,13 +227,11 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
       }
   }
   __syncthreads();
-  // 4. second transform at half length: Z = FFT_{N/2}(z), left in LDS in
-  // natural order (elements 0 .. N/2-1 of the buffer)
-  p2_passes<T, LOGN, true, 0, p2_half_plan(LOGN, F64).np>(buf, tw, u);
-
-  // 5. prefix sums psum(t + 1) of |F(t) - mean|^2 from a second
-  // (cache-resident) read of the series into the free second half of the
-  // buffer; first and last kEdge samples into `edge`.
+  // 4. prefix sums psum(t + 1) of |F(t) - mean|^2 from a second
+  // (cache-resident) read of the series into the second half of the buffer,
+  // which the half-length transform leaves free; first and last kEdge
+  // samples into `edge`.  Done before that transform so the re-read follows
+  // the first read closely.
   Acc* const tail = reinterpret_cast<Acc*>(buf + padi(NH));
   // each thread: a contiguous segment of seg <= SEGMAX samples, local running
   // sums, then one exclusive scan over the group's threads; seg is odd, so
@@ -263,19 +262,19 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
   Acc exc = wave::from_prev_lane(inc);  // 0 in lane 0
   if (lane == 63) chunk[wg] = inc;
   __syncthreads();
-  Acc ptot = 0;
 #pragma unroll
-  for (int w = 0; w < WPT; ++w) {
-    const Acc x = chunk[w];
-    if (w < wg) exc += x;
-    ptot += x;
-  }
+  for (int w = 0; w < WPT; ++w)
+    if (w < wg) exc += chunk[w];
 #pragma unroll
   for (int i = 0; i < SEGMAX; ++i) {
     const int t = ts + i;
     if (i < seg && t < L - 1) tail[t] = pre[i] + exc;
   }
   __syncthreads();
+
+  // 5. second transform at half length: Z = FFT_{N/2}(z), left in LDS in
+  // natural order (elements 0 .. N/2-1 of the buffer)
+  p2_passes<T, LOGN, true, 0, p2_half_plan(LOGN, F64).np>(buf, tw, u);
 
   // 6. lags: thread tid serves q group h = tid % G and lag indices
   // nl - 1 - tid / G, stepping by NT / G, so neighbouring lanes store the same
@@ -325,6 +324,9 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
     }
   }
   if (active && u == 0) {
+    Acc ptot = 0;
+#pragma unroll
+    for (int w = 0; w < WPT; ++w) ptot += chunk[w];
     const int ky = q / a.nxh, kx = q - ky * a.nxh;
     const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
     const Acc mr2 = static_cast<Acc>(fsr / L), mi2 = static_cast<Acc>(fsi / L);
