@@ -16,6 +16,7 @@ the q grid); the (rows, ny, nxh) data are reduced by the HIP kernels of
 
 import ctypes
 import hashlib
+import threading
 import warnings
 from collections import OrderedDict
 from dataclasses import dataclass
@@ -167,6 +168,7 @@ def _reduce(ptr, f32, rows, ny, nxh, bin_index, w, w2, nbins, eval_err):
 
 _MAPS_CACHE: "OrderedDict[bytes, tuple]" = OrderedDict()
 _MAPS_CACHE_SIZE = 8
+_MAPS_LOCK = threading.Lock()
 
 
 def _maps_key(kx, ky, width, bins, range, mask, weights, eval_err) -> Optional[bytes]:
@@ -191,9 +193,11 @@ def _device_maps(kx, ky, width, ny, nxh, bins, range, mask, weights, eval_err):
     """Bin maps of the half-plane (ny, nxh) grid, kept for the last few
     geometries: repeated averages of results of one size reuse them."""
     key = _maps_key(kx, ky, width, bins, range, mask, weights, eval_err)
-    if key is not None and key in _MAPS_CACHE:
-        _MAPS_CACHE.move_to_end(key)
-        return _MAPS_CACHE[key]
+    if key is not None:
+        with _MAPS_LOCK:
+            if key in _MAPS_CACHE:
+                _MAPS_CACHE.move_to_end(key)
+                return _MAPS_CACHE[key]
     counts = _half_plane_counts((ny, nxh), width)
     maps = bin_maps(_k_modulus(kx, ky), bins, range, mask, weights, counts, eval_err)
     if np.any(np.diff(maps[0], axis=1) < 0):
@@ -201,9 +205,10 @@ def _device_maps(kx, ky, width, ny, nxh, bins, range, mask, weights, eval_err):
     for a in maps:
         a.setflags(write=False)
     if key is not None:
-        _MAPS_CACHE[key] = maps
-        while len(_MAPS_CACHE) > _MAPS_CACHE_SIZE:
-            _MAPS_CACHE.popitem(last=False)
+        with _MAPS_LOCK:
+            _MAPS_CACHE[key] = maps
+            while len(_MAPS_CACHE) > _MAPS_CACHE_SIZE:
+                _MAPS_CACHE.popitem(last=False)
     return maps
 
 
