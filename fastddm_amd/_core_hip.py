@@ -52,6 +52,9 @@ def lib() -> ctypes.CDLL:
         "fddm_set_device": [i32, cp, sz],
         "fddm_get_device": [ctypes.POINTER(i32), cp, sz],
         "fddm_get_free_device_memory": [ctypes.POINTER(u64), cp, sz],
+        "fddm_get_free_host_memory": [ctypes.POINTER(u64), cp, sz],
+        "fddm_get_host_memory": [i32, u64, u64, u64, u64, i32, ctypes.POINTER(u64), cp, sz],
+        "fddm_check_host_memory": [i32, u64, u64, u64, u64, i32, ctypes.POINTER(i32), cp, sz],
         "fddm_ddm_fft": [p, i32, u64, u64, u64, p, u64, u64, u64, u64, p, i32, i32, i32, p, cp, sz],
         "fddm_ddm_diff": [p, i32, u64, u64, u64, p, u64, u64, u64, p, i32, i32, i32, p, cp, sz],
         "fddm_release_cache": [cp, sz],
@@ -106,6 +109,9 @@ EXPORTED_SYMBOLS = (
     "fddm_set_device",
     "fddm_get_device",
     "fddm_get_free_device_memory",
+    "fddm_get_free_host_memory",
+    "fddm_get_host_memory",
+    "fddm_check_host_memory",
     "fddm_ddm_fft",
     "fddm_ddm_diff",
     "fddm_release_cache",
@@ -178,6 +184,49 @@ def get_free_device_memory() -> int:
     return f.value
 
 
+def get_free_host_memory() -> int:
+    """Available host memory in bytes (reference get_free_host_memory,
+    src/cuda/memchk_gpu.cc:35-87)."""
+    f = ctypes.c_uint64(0)
+    _call("fddm_get_free_host_memory", ctypes.byref(f))
+    return f.value
+
+
+def _host_memory(mode: int, nx, ny, length, num_lags, out_dtype) -> int:
+    b = ctypes.c_uint64(0)
+    _call("fddm_get_host_memory", mode, nx, ny, length, num_lags, _out_code(out_dtype), ctypes.byref(b))
+    return b.value
+
+
+def get_host_memory_fft(nx, ny, length, num_lags, out_dtype=np.float64) -> int:
+    """Host bytes an fft-mode call needs besides its input: the result and
+    the pinned staging buffers (reference get_host_memory_fft,
+    memchk_gpu.cc:116-140)."""
+    return _host_memory(MODE_FFT, nx, ny, length, num_lags, out_dtype)
+
+
+def get_host_memory_diff(nx, ny, length, num_lags, out_dtype=np.float64) -> int:
+    """As get_host_memory_fft, for diff mode (memchk_gpu.cc:89-114)."""
+    return _host_memory(MODE_DIFF, nx, ny, length, num_lags, out_dtype)
+
+
+def _check_host_memory(mode: int, nx, ny, length, num_lags, out_dtype) -> bool:
+    ok = ctypes.c_int(0)
+    _call("fddm_check_host_memory", mode, nx, ny, length, num_lags, _out_code(out_dtype), ctypes.byref(ok))
+    return bool(ok.value)
+
+
+def check_host_memory_fft(nx, ny, length, num_lags, out_dtype=np.float64) -> bool:
+    """True when 0.95 x the available host memory exceeds that need
+    (reference check_host_memory_fft_py, memchk_gpu.cc:172-190)."""
+    return _check_host_memory(MODE_FFT, nx, ny, length, num_lags, out_dtype)
+
+
+def check_host_memory_diff(nx, ny, length, num_lags, out_dtype=np.float64) -> bool:
+    """As check_host_memory_fft, for diff mode (memchk_gpu.cc:142-160)."""
+    return _check_host_memory(MODE_DIFF, nx, ny, length, num_lags, out_dtype)
+
+
 def release_cache() -> None:
     _call("fddm_release_cache")
 
@@ -226,8 +275,10 @@ def ddm_fft(img_seq, lags, nx, ny, nt, window, precision="fp64", out_dtype=np.fl
     img = _img_arg(img_seq)
     lg = _lags_arg(lags)
     wp, wf32, _keep = _window_arg(window)
-    out = np.empty((len(lg) + 2, ny, nx // 2 + 1), dtype=out_dtype)
     L, H, W = img.shape
+    if not check_host_memory_fft(nx, ny, L, len(lg), out_dtype):
+        raise RuntimeError("Not enough space in memory to store the result.\n")
+    out = np.empty((len(lg) + 2, ny, nx // 2 + 1), dtype=out_dtype)
     _call(
         "fddm_ddm_fft", img.ctypes.data, DTYPE_CODES[img.dtype], L, H, W, lg.ctypes.data, len(lg),
         nx, ny, nt, wp, wf32, _prec_code(precision), _out_code(out_dtype), out.ctypes.data,
@@ -240,8 +291,10 @@ def ddm_diff(img_seq, lags, nx, ny, window, precision="fp64", out_dtype=np.float
     img = _img_arg(img_seq)
     lg = _lags_arg(lags)
     wp, wf32, _keep = _window_arg(window)
-    out = np.empty((len(lg) + 2, ny, nx // 2 + 1), dtype=out_dtype)
     L, H, W = img.shape
+    if not check_host_memory_diff(nx, ny, L, len(lg), out_dtype):
+        raise RuntimeError("Not enough space in memory to store the result.\n")
+    out = np.empty((len(lg) + 2, ny, nx // 2 + 1), dtype=out_dtype)
     _call(
         "fddm_ddm_diff", img.ctypes.data, DTYPE_CODES[img.dtype], L, H, W, lg.ctypes.data, len(lg),
         nx, ny, wp, wf32, _prec_code(precision), _out_code(out_dtype), out.ctypes.data,

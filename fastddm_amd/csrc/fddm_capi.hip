@@ -658,6 +658,47 @@ int fddm_get_free_device_memory(uint64_t* free_bytes, char* err, size_t errlen) 
   return FDDM_OK;
 }
 
+int fddm_get_free_host_memory(uint64_t* free_bytes, char* err, size_t errlen) {
+  if (!free_bytes) return set_err(err, errlen, FDDM_ERR_ARG, "NULL free_bytes");
+  FILE* f = fopen("/proc/meminfo", "r");
+  if (!f) return set_err(err, errlen, FDDM_ERR_ARG, "Failed to retrieve memory information.");
+  char line[256];
+  unsigned long long kib = 0;
+  bool found = false;
+  while (fgets(line, sizeof line, f))
+    if (sscanf(line, "MemAvailable: %llu kB", &kib) == 1) {
+      found = true;
+      break;
+    }
+  fclose(f);
+  if (!found) return set_err(err, errlen, FDDM_ERR_ARG, "Failed to retrieve memory information.");
+  *free_bytes = static_cast<uint64_t>(kib) * 1024;
+  return FDDM_OK;
+}
+
+int fddm_get_host_memory(int mode, uint64_t nx, uint64_t ny, uint64_t length, uint64_t num_lags, int out_dtype,
+                         uint64_t* bytes, char* err, size_t errlen) {
+  (void)length;  // the frames stay in the caller's array; the spectra never come to the host
+  if (!bytes) return set_err(err, errlen, FDDM_ERR_ARG, "NULL bytes");
+  if (mode != MODE_FFT && mode != MODE_DIFF) return set_err(err, errlen, FDDM_ERR_ARG, "unknown mode %d", mode);
+  if (out_dtype != OUT_F64 && out_dtype != OUT_F32)
+    return set_err(err, errlen, FDDM_ERR_ARG, "unknown output dtype %d", out_dtype);
+  const uint64_t scalar = out_dtype == OUT_F32 ? 4 : 8;
+  *bytes = (nx / 2 + 1) * ny * (num_lags + 2) * scalar + 2 * static_cast<uint64_t>(kStageBytes);
+  return FDDM_OK;
+}
+
+int fddm_check_host_memory(int mode, uint64_t nx, uint64_t ny, uint64_t length, uint64_t num_lags, int out_dtype,
+                           int* ok, char* err, size_t errlen) {
+  if (!ok) return set_err(err, errlen, FDDM_ERR_ARG, "NULL ok");
+  uint64_t need = 0, avail = 0;
+  int rc = fddm_get_host_memory(mode, nx, ny, length, num_lags, out_dtype, &need, err, errlen);
+  if (!rc) rc = fddm_get_free_host_memory(&avail, err, errlen);
+  if (rc) return rc;
+  *ok = static_cast<uint64_t>(0.95 * static_cast<double>(avail)) > need ? 1 : 0;  // the reference's margin
+  return FDDM_OK;
+}
+
 int fddm_ddm_fft(const void* img, int dtype, uint64_t length, uint64_t height, uint64_t width,
                  const uint32_t* lags, uint64_t num_lags, uint64_t nx, uint64_t ny, uint64_t nt,
                  const void* window, int window_f32, int precision, int out_dtype, void* out, char* err,

@@ -67,6 +67,20 @@ int fddm_set_device(int gpu_id, char *err, size_t errlen);
 int fddm_get_device(int *gpu_id, char *err, size_t errlen);
 /* Free device memory in bytes (get_free_device_memory, gpu_utils.cu:83-116). */
 int fddm_get_free_device_memory(uint64_t *free_bytes, char *err, size_t errlen);
+/* Available host memory in bytes, MemAvailable of /proc/meminfo
+ * (get_free_host_memory, src/cuda/memchk_gpu.cc:35-87). */
+int fddm_get_free_host_memory(uint64_t *free_bytes, char *err, size_t errlen);
+/* Host memory a fddm_ddm_fft / fddm_ddm_diff call needs besides its input:
+ * the (num_lags+2, ny, nx/2+1) result in the output dtype plus the two pinned
+ * staging buffers (get_host_memory_fft / get_host_memory_diff,
+ * memchk_gpu.cc:89-140, whose estimate also holds the host-staged spectra
+ * of the CUDA core).  mode: FDDM_MODE_FFT or FDDM_MODE_DIFF. */
+int fddm_get_host_memory(int mode, uint64_t nx, uint64_t ny, uint64_t length, uint64_t num_lags,
+                         int out_dtype, uint64_t *bytes, char *err, size_t errlen);
+/* *ok = 1 when 0.95 x the available host memory exceeds that need
+ * (check_host_memory_fft_py / check_host_memory_diff_py, memchk_gpu.cc:142-199). */
+int fddm_check_host_memory(int mode, uint64_t nx, uint64_t ny, uint64_t length, uint64_t num_lags,
+                           int out_dtype, int *ok, char *err, size_t errlen);
 
 /* ---- whole path, host buffers in and out ----
  * Replaces ddm_fft_cuda<T> (ddm_cuda.h:52-58, ddm_cuda.cc:181-285) and

@@ -77,3 +77,24 @@ def test_plan_argument_errors_are_reported(lib, over, code, text):
     rc, msg = _plan_create(lib, **over)
     assert rc == code
     assert text in msg
+
+
+def test_host_memory_utilities(lib):
+    """get_free_host_memory / get_host_memory_* / check_host_memory_* (the
+    reference's device utilities, src/cuda/memchk_gpu.cc:35-199) are host-only
+    and answer without a GPU."""
+    from fastddm_amd import _core_hip as hip
+
+    with open("/proc/meminfo") as f:
+        avail = next(int(l.split()[1]) * 1024 for l in f if l.startswith("MemAvailable:"))
+    got = hip.get_free_host_memory()
+    assert abs(got - avail) < 0.25 * avail  # the two reads are moments apart
+    stage = 2 * (64 << 20)
+    assert hip.get_host_memory_fft(512, 512, 2000, 1999) == 257 * 512 * 2001 * 8 + stage
+    assert hip.get_host_memory_diff(512, 512, 300, 50, np.float32) == 257 * 512 * 52 * 4 + stage
+    assert hip.check_host_memory_fft(64, 64, 100, 99)
+    assert not hip.check_host_memory_diff(1 << 20, 1 << 20, 100, 1 << 20)
+    err = ctypes.create_string_buffer(256)
+    b = ctypes.c_uint64(0)
+    assert lib.fddm_get_host_memory(7, 8, 8, 4, 3, 0, ctypes.byref(b), err, 256) != 0
+    assert b"unknown mode" in err.value
