@@ -178,6 +178,18 @@ def test_every_time_plan_matches_oracle(hip, L, N, precision):
     CHECK[precision](out, ref)
 
 
+@pytest.mark.parametrize("L", [700, 1500])
+def test_time_plan_odd_number_of_q(hip, L):
+    """45 q-series (9 x 8 images): the plans that put several q in one
+    workgroup (two at N = 4096, fp32c) end with a partly empty workgroup
+    whose idle groups still serve the lag loop of the others."""
+    img = ar1_speckle(L, 9, 8, sigma=1.1, diffusivity=0.08, mean=700, std=110, seed=L + 7)
+    lags = np.arange(1, L)
+    ref = onp.ddm_fft(img, lags, 8, 9, 2 * L, None, subtract_mean=True)
+    out = hip.ddm_fft(img, lags, 8, 9, next_fast_len(2 * L, "hip"), None, precision="fp32c")
+    CHECK["fp32c"](out, ref)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32c"])
 @pytest.mark.parametrize("L", [120, 200, 1500, 5000])
 def test_diff_mode_long_series(hip, L, precision):
