@@ -339,9 +339,9 @@ int fddm::plan_create_ex(fddm_plan** out, int mode, int dtype, uint64_t L, uint6
     if (p2_spatial_ok(ly) && p->Tb >= 4) p->logn_y = ly;
   }
   int rc;
-  auto upload_sp = [&](void** dst, int logn) {
+  auto upload_sp = [&](void** dst, int logn, bool cols) {
     int r0, r1, r2;
-    p2_spatial_plan(logn, &r0, &r1, &r2);
+    p2_spatial_plan(logn, cols, &r0, &r1, &r2);
     if (f64) {
       auto w = sp_twiddles<double>(r0, r1, r2);
       return upload(dst, w.data(), w.size() * sizeof(double), err, errlen);
@@ -349,9 +349,9 @@ int fddm::plan_create_ex(fddm_plan** out, int mode, int dtype, uint64_t L, uint6
     auto w = sp_twiddles<float>(r0, r1, r2);
     return upload(dst, w.data(), w.size() * sizeof(float), err, errlen);
   };
-  if ((rc = p->logn_x ? upload_sp(&p->d_twx, p->logn_x) : upload_twiddles(&p->d_twx, p->dx, f64, err, errlen)))
+  if ((rc = p->logn_x ? upload_sp(&p->d_twx, p->logn_x, false) : upload_twiddles(&p->d_twx, p->dx, f64, err, errlen)))
     return fail(rc);
-  if ((rc = p->logn_y ? upload_sp(&p->d_twy, p->logn_y) : upload_twiddles(&p->d_twy, p->dy, f64, err, errlen)))
+  if ((rc = p->logn_y ? upload_sp(&p->d_twy, p->logn_y, true) : upload_twiddles(&p->d_twy, p->dy, f64, err, errlen)))
     return fail(rc);
   if (mode == MODE_FFT) {
     if (f64) {

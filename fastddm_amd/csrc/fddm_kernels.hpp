@@ -61,7 +61,7 @@ size_t p2_lds_bytes(int logn, int precision);
 hipError_t launch_time_p2(const TimeArgs& a, int logn, int precision, int out_dtype, hipStream_t s);
 // Power-of-two spatial kernels (256 <= n <= 4096): plan radices and launchers.
 bool p2_spatial_ok(int logn);
-int p2_spatial_plan(int logn, int* r0, int* r1, int* r2);
+int p2_spatial_plan(int logn, bool cols, int* r0, int* r1, int* r2);
 hipError_t launch_rows_p2(const RowArgs& a, int precision, hipStream_t s);
 hipError_t launch_cols_p2(const ColArgs& a, int precision, hipStream_t s);
 hipError_t launch_fft_test(void* data, int n, int batch, const FftDesc& d, const void* tw,

@@ -210,13 +210,16 @@ struct Pass {
 };
 
 // Three-pass plans for power-of-two spatial transforms (radices in order).
+// Columns of 1024 start with radix 8 so their first pass, read straight from
+// HBM, has a butterfly for every thread (measured faster there; rows keep
+// radix 16 first).
 struct SpPlan {
   int r0, r1, r2;
 };
-__host__ __device__ constexpr SpPlan sp_plan(int logn) {
+__host__ __device__ constexpr SpPlan sp_plan(int logn, bool cols = false) {
   return logn == 8    ? SpPlan{8, 8, 4}
          : logn == 9  ? SpPlan{8, 8, 8}
-         : logn == 10 ? SpPlan{16, 8, 8}
+         : logn == 10 ? (cols ? SpPlan{8, 16, 8} : SpPlan{16, 8, 8})
          : logn == 11 ? SpPlan{16, 16, 8}
                       : SpPlan{16, 16, 16};
 }

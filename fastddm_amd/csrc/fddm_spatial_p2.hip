@@ -203,7 +203,7 @@ template <typename T, int LOGY, int TB, int NT>
 __global__ void __launch_bounds__(NT) k_cols_p2(ColArgs a) {
   using C = cplx_t<T>;
   constexpr int NY = 1 << LOGY;
-  constexpr SpPlan P = sp_plan(LOGY);
+  constexpr SpPlan P = sp_plan(LOGY, true);
   using A0 = Pass<T, NY, P.r0, 1, NT, TB, true, TB, 1>;
   using A1 = Pass<T, NY, P.r1, P.r0, NT, TB, true, TB, 1>;
   using A2 = Pass<T, NY, P.r2, P.r0 * P.r1, NT, TB, true, TB, 1>;
@@ -216,11 +216,11 @@ __global__ void __launch_bounds__(NT) k_cols_p2(ColArgs a) {
   const C* __restrict__ src = static_cast<const C*>(a.R) + (tb * nxh + kx) * static_cast<size_t>(H) * TB;
   const int nin = H * TB;
   const C* __restrict__ tw = static_cast<const C*>(a.twy);
-  if constexpr (sizeof(T) == 4 && LOGY <= 9) {
+  if constexpr (sizeof(T) == 4 && LOGY <= 10) {
     // first pass straight from the slab: butterfly j of transform ti takes
     // y = j + r NY/R0, element y TB + ti (rows y >= H are the zero padding);
     // neighbouring lanes read neighbouring elements.  (Measured faster for
-    // FP32 columns up to 512; longer columns, whose first pass is radix 16,
+    // FP32 columns up to 1024, whose first pass is radix 8; longer columns
     // and FP64 keep the staged copy below.)
     constexpr int M0 = NY / P.r0;
     C v[A0::MB][P.r0];
@@ -384,9 +384,9 @@ bool p2_spatial_ok(int logn) { return logn >= 8 && logn <= 12; }
 
 // Twiddles of the spatial power-of-two plan: passes 1 and 2 of sp_plan,
 // [k][r] = exp(-2 pi i r k / (NS R)).
-int p2_spatial_plan(int logn, int* r0, int* r1, int* r2) {
+int p2_spatial_plan(int logn, bool cols, int* r0, int* r1, int* r2) {
   if (!p2_spatial_ok(logn)) return 0;
-  const SpPlan P = sp_plan(logn);
+  const SpPlan P = sp_plan(logn, cols);
   *r0 = P.r0;
   *r1 = P.r1;
   *r2 = P.r2;
