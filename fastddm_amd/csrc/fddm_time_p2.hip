@@ -17,11 +17,13 @@
 //      formed there and written to LDS as z(n) = P(2n) + i P(2n+1);
 //   4. re-reads the series (cache resident) for the prefix sums of |F(t)|^2
 //      (double on the compensated paths; contiguous segments per thread and
-//      one group scan) into the buffer's second half, and keeps the first and
+//      one group scan) into one half of the buffer, and keeps the first and
 //      last kEdge samples;
 //   5. the second forward transform of the real P runs as one complex
-//      transform of length N/2 of z (p2_half_plan) in the first half; for
-//      each lag Re FFT_N(P)(d) = N C(d) is untangled from Z(d) and Z(N/2 - d);
+//      transform of length N/2 of z (p2_half_plan) in the other half (in
+//      place, or alternating between the halves before step 4 on the paired
+//      N = 4096 plan); for each lag Re FFT_N(P)(d) = N C(d) is untangled
+//      from Z(d) and Z(N/2 - d);
 //   6. evaluates D for every requested lag (direct sums when L - dt <= kEdge),
 //      PS and VAR, and stores them fftshifted along y.
 #include <type_traits>
@@ -122,6 +124,26 @@ __device__ __forceinline__ void p2_passes(cplx_t<T>* buf, const cplx_t<T>* __res
     Ps::store(buf, v, u);
     __syncthreads();
     p2_passes<T, LOGN, HALF, I + 1, LAST>(buf, tw, u);
+  }
+}
+
+// Passes I .. LAST-1 of the half-length transform, alternating between the
+// buffer halves a and b: one barrier per pass instead of two.
+template <typename T, int LOGN, int I, int LAST>
+__device__ __forceinline__ void p2_half_pp(cplx_t<T>* a, cplx_t<T>* b, const cplx_t<T>* __restrict__ tw, int u) {
+  constexpr bool F64 = sizeof(T) == 8;
+  constexpr P2Plan P = p2_half_plan(LOGN, F64);
+  if constexpr (I < LAST) {
+    constexpr int N = 1 << (LOGN - 1);
+    constexpr int R = p2_radix(P, I);
+    constexpr int NS = p2_ns(P, I);
+    constexpr int OFF = p2_tw_h(LOGN, F64, I);
+    using Ps = Pass<T, N, R, NS, P.threads>;
+    cplx_t<T> v[Ps::MB][R];
+    Ps::compute(a, v, tw + OFF, u);
+    Ps::store(b, v, u);
+    __syncthreads();
+    p2_half_pp<T, LOGN, I + 1, LAST>(b, a, tw, u);
   }
 }
 
@@ -227,54 +249,65 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
       }
   }
   __syncthreads();
-  // 4. prefix sums psum(t + 1) of |F(t) - mean|^2 from a second
-  // (cache-resident) read of the series into the second half of the buffer,
-  // which the half-length transform leaves free; first and last kEdge
-  // samples into `edge`.  Done before that transform so the re-read follows
-  // the first read closely.
-  Acc* const tail = reinterpret_cast<Acc*>(buf + padi(NH));
-  // each thread: a contiguous segment of seg <= SEGMAX samples, local running
-  // sums, then one exclusive scan over the group's threads; seg is odd, so
-  // the 8-byte stores of neighbouring threads fall in different LDS banks
-  constexpr int SEGMAX = ((N / 2 + TPT - 1) / TPT) | 1;
-  const int seg = ((L + TPT - 1) / TPT) | 1;
-  const int ts = u * seg;
-  Acc pre[SEGMAX];
-  Acc run = 0;
+  // 4-5. prefix sums psum(t + 1) of |F(t) - mean|^2 from a second
+  // (cache-resident) read of the series into the buffer half the second
+  // transform leaves free, first and last kEdge samples into `edge`; and the
+  // second transform at half length, Z = FFT_{N/2}(z).  PP (the paired
+  // N = 4096 plan): the transform alternates between the halves (one barrier
+  // per pass) and the sums follow it into the half it leaves; otherwise the
+  // sums come first, so the re-read follows the first read closely, and the
+  // transform runs in place in the first half.  Both orders were measured.
+  constexpr bool PP = Lay::kPair;
+  constexpr int NPH = p2_half_plan(LOGN, F64).np;
+  constexpr bool ZA = !PP || NPH % 2 == 0;  // Z ends in the first half
+  auto prefix_sums = [&](Acc* const tail) {
+    // each thread: a contiguous segment of seg <= SEGMAX samples, local running
+    // sums, then one exclusive scan over the group's threads; seg is odd, so
+    // the 8-byte stores of neighbouring threads fall in different LDS banks
+    constexpr int SEGMAX = ((N / 2 + TPT - 1) / TPT) | 1;
+    const int seg = ((L + TPT - 1) / TPT) | 1;
+    const int ts = u * seg;
+    Acc pre[SEGMAX];
+    Acc run = 0;
 #pragma unroll
-  for (int i = 0; i < SEGMAX; ++i) {
-    const int t = ts + i;
-    Acc val = 0;
-    if (i < seg && active && t < L) {
-      C f = src[t];
-      f.x -= m_r;
-      f.y -= m_i;
-      if constexpr (SUB) {
-        if (t < kEdge) edge[t] = f;
-        if (t >= L - kEdge) edge[t - L + 2 * kEdge] = f;
+    for (int i = 0; i < SEGMAX; ++i) {
+      const int t = ts + i;
+      Acc val = 0;
+      if (i < seg && active && t < L) {
+        C f = src[t];
+        f.x -= m_r;
+        f.y -= m_i;
+        if constexpr (SUB) {
+          if (t < kEdge) edge[t] = f;
+          if (t >= L - kEdge) edge[t - L + 2 * kEdge] = f;
+        }
+        val = static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
       }
-      val = static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
+      run += val;
+      pre[i] = run;
     }
-    run += val;
-    pre[i] = run;
-  }
-  const Acc inc = wave::incl_scan(run);
-  Acc exc = wave::from_prev_lane(inc);  // 0 in lane 0
-  if (lane == 63) chunk[wg] = inc;
-  __syncthreads();
+    const Acc inc = wave::incl_scan(run);
+    Acc exc = wave::from_prev_lane(inc);  // 0 in lane 0
+    if (lane == 63) chunk[wg] = inc;
+    __syncthreads();
 #pragma unroll
-  for (int w = 0; w < WPT; ++w)
-    if (w < wg) exc += chunk[w];
+    for (int w = 0; w < WPT; ++w)
+      if (w < wg) exc += chunk[w];
 #pragma unroll
-  for (int i = 0; i < SEGMAX; ++i) {
-    const int t = ts + i;
-    if (i < seg && t < L - 1) tail[t] = pre[i] + exc;
+    for (int i = 0; i < SEGMAX; ++i) {
+      const int t = ts + i;
+      if (i < seg && t < L - 1) tail[t] = pre[i] + exc;
+    }
+    __syncthreads();
+  };
+  Acc* const tail = reinterpret_cast<Acc*>(ZA ? buf + padi(NH) : buf);
+  if constexpr (PP) {
+    p2_half_pp<T, LOGN, 0, NPH>(buf, buf + padi(NH), tw, u);
+    prefix_sums(tail);
+  } else {
+    prefix_sums(tail);
+    p2_passes<T, LOGN, true, 0, NPH>(buf, tw, u);
   }
-  __syncthreads();
-
-  // 5. second transform at half length: Z = FFT_{N/2}(z), left in LDS in
-  // natural order (elements 0 .. N/2-1 of the buffer)
-  p2_passes<T, LOGN, true, 0, p2_half_plan(LOGN, F64).np>(buf, tw, u);
 
   // 6. lags: thread tid serves q group h = tid % G and lag indices
   // nl - 1 - tid / G, stepping by NT / G, so neighbouring lanes store the same
@@ -286,8 +319,9 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
   {
     const int h = tid % G, v = tid / G, qh = qbase + h;
     if (qh < nq) {
-      const C* const hbuf = buf0 + h * lds_padded_size(N);
-      const Acc* const htail = reinterpret_cast<const Acc*>(hbuf + padi(NH));
+      const C* const hgrp = buf0 + h * lds_padded_size(N);
+      const C* const hbuf = ZA ? hgrp : hgrp + padi(NH);  // Z
+      const Acc* const htail = reinterpret_cast<const Acc*>(ZA ? hgrp + padi(NH) : hgrp);
       const C* const hedge = edge0 + h * 2 * kEdge;
       Acc hptot = 0;
 #pragma unroll
