@@ -24,6 +24,27 @@ using namespace p2;
 // of resident workgroups, so each one brings 8 waves.
 constexpr int kThreads = 512;
 
+// Item of this workgroup.  When a frame block's run of TB values is shorter
+// than a 128-byte line (8 FP32 frames: 64 bytes), neighbouring items write
+// the two halves of the same lines; the XCD-aware order puts them on one XCD
+// so the halves meet in its L2 instead of reaching memory as partial lines
+// from two XCDs.  Workgroups are dealt to the 8 XCDs round-robin (block b
+// runs on XCD b mod 8) and the grid is a multiple of 8, so item
+// (b mod 8) (grid / 8) + b / 8 gives each XCD a contiguous run of items.
+// Full-line runs keep the launch order (measured neutral to 1.5 % slower
+// with the XCD order).
+template <bool XCD>
+__device__ __forceinline__ uint32_t block_item() {
+  if constexpr (XCD) return (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  return blockIdx.x;
+}
+template <typename T, int TB>
+constexpr bool kPartialLines = TB * 2 * sizeof(T) < 128;
+
+// Launch grid for `items` workgroups: rounded up to a multiple of 8 (the
+// blocks past `items` return at once).
+inline unsigned xcd_grid(unsigned items) { return (items + 7u) & ~7u; }
+
 // Pixel (p, x) of a pair row: converted, frame 0 subtracted in integer
 // arithmetic where the type allows, times the window value.
 template <typename Tin, typename T>
@@ -40,7 +61,7 @@ __device__ __forceinline__ T pixel_value(Tin v, Tin v0, bool sub, T w) {
 }
 
 template <typename Tin, typename T, int LOGX, int NP>
-__global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a) {
+__global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems) {
   using C = cplx_t<T>;
   constexpr int NX = 1 << LOGX, NXH = NX / 2 + 1, TB = 2 * NP;
   constexpr SpPlan P = sp_plan(LOGX);
@@ -48,8 +69,10 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a) {
   C* const buf = reinterpret_cast<C*>(smem);
   const int tid = threadIdx.x;
   const int H = a.H, W = a.W;
-  const int y = blockIdx.x % H;
-  const uint64_t tb = blockIdx.x / H;
+  const uint32_t item = block_item<kPartialLines<T, TB>>();  // tb * H + y
+  if (item >= nitems) return;
+  const int y = static_cast<int>(item % H);
+  const uint64_t tb = item / H;
   const uint64_t t0 = tb * TB;
   const Tin* __restrict__ img = static_cast<const Tin*>(a.img);
   const T* __restrict__ win = static_cast<const T*>(a.window);
@@ -200,7 +223,7 @@ __global__ void __launch_bounds__(kThreads, 8) k_rows_p2_pf(RowArgs a, uint32_t 
 }
 
 template <typename T, int LOGY, int TB, int NT>
-__global__ void __launch_bounds__(NT) k_cols_p2(ColArgs a) {
+__global__ void __launch_bounds__(NT) k_cols_p2(ColArgs a, uint32_t nitems) {
   using C = cplx_t<T>;
   constexpr int NY = 1 << LOGY;
   constexpr SpPlan P = sp_plan(LOGY, true);
@@ -211,8 +234,15 @@ __global__ void __launch_bounds__(NT) k_cols_p2(ColArgs a) {
   C* const buf = reinterpret_cast<C*>(smem);
   const int tid = threadIdx.x;
   const int nxh = a.nxh, H = a.H;
-  const int kx = blockIdx.x % nxh;
-  const uint64_t tb = blockIdx.x / nxh;
+  // partial-line runs: item = kx * nblocks + tb, so the frame blocks of one
+  // kx, whose S runs share lines, are neighbours on one XCD; otherwise
+  // item = tb * nxh + kx
+  constexpr bool XCD = kPartialLines<T, TB>;
+  const uint32_t item = block_item<XCD>();
+  if (item >= nitems) return;
+  const uint32_t nb = static_cast<uint32_t>(a.nblocks);
+  const int kx = static_cast<int>(XCD ? item / nb : item % static_cast<uint32_t>(nxh));
+  const uint64_t tb = XCD ? item - static_cast<uint32_t>(kx) * nb : item / static_cast<uint32_t>(nxh);
   const C* __restrict__ src = static_cast<const C*>(a.R) + (tb * nxh + kx) * static_cast<size_t>(H) * TB;
   const int nin = H * TB;
   const C* __restrict__ tw = static_cast<const C*>(a.twy);
@@ -303,7 +333,7 @@ hipError_t rows_np(const RowArgs& a, unsigned grid, hipStream_t s) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL(kernel, dim3(xcd_grid(grid)), dim3(kThreads), lds, s, a, static_cast<uint32_t>(grid));
   return hipGetLastError();
 }
 
@@ -352,7 +382,7 @@ hipError_t cols_one(const ColArgs& a, unsigned grid, hipStream_t s) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(NT), lds, s, a);
+  hipLaunchKernelGGL(kernel, dim3(xcd_grid(grid)), dim3(NT), lds, s, a, static_cast<uint32_t>(grid));
   return hipGetLastError();
 }
 

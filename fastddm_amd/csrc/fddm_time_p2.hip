@@ -26,6 +26,12 @@
 //      from Z(d) and Z(N/2 - d);
 //   6. evaluates D for every requested lag (direct sums when L - dt <= kEdge),
 //      PS and VAR, and stores them fftshifted along y.
+//
+// The output is lag-major, so each lag store of one q writes one value into
+// its own line.  Workgroups of several q interleave them across lanes; the
+// compensated FP32 plans from N = 8192, whose LDS holds a single q, instead
+// run steps 1-6 for two neighbouring output columns in turn, keep the first
+// column's D in registers and store each lag's two values together.
 #include <type_traits>
 #include <utility>
 
@@ -78,6 +84,19 @@ __device__ __forceinline__ void group_sum2(double& x, double& y, double* red) {
   y = sy;
 }
 
+// q of output column o: the outputs are fftshifted along y, so column o sits
+// in row (ky + ny/2) mod ny.
+__device__ __forceinline__ int q_of_column(int o, int ny, int nxh) {
+  const int yo = o / nxh, kx = o - yo * nxh;
+  return ((yo + ny - ny / 2) % ny) * nxh + kx;
+}
+
+// Two neighbouring output values, stored as one access.
+template <typename O>
+struct alignas(2 * sizeof(O)) OutPair {
+  O first, second;
+};
+
 // Workgroup shape and LDS layout of the time kernel at N = 2^LOGN.  Plans of
 // fewer than kMinThreads threads put G = kMinThreads / TPT q in a workgroup;
 // compensated plans of 256 or 512 threads put two q in one when two such
@@ -97,7 +116,15 @@ struct P2Lds {
   static constexpr int NT = TPT < kMinThreads ? kMinThreads : (kPair ? 2 * TPT : TPT);  // workgroup size
   static constexpr int G = NT / TPT;
   // paired workgroups: two per CU hold 8 waves per SIMD when at most 64 VGPRs are used
-  static constexpr int kMinWaves = kPair ? 8 : 1;
+  // Compensated FP32 plans from N = 8192 hold one q per workgroup (the LDS
+  // has no room for a second) and leave registers free: the workgroup
+  // transforms the q of two neighbouring output columns one after the other
+  // and keeps the first one's D in registers (KMAX lag iterations per
+  // thread), so each lag store writes both columns (16 bytes) at once.
+  static constexpr int SEQ = (SUB && sizeof(T) == 4 && !kPair && TPT >= kMinThreads && LOGN >= 13) ? 2 : 1;
+  static constexpr int KMAX = (N / 2 + NT - 1) / NT;
+  // registers for the 4 waves per SIMD the LDS allows
+  static constexpr int kMinWaves = kPair ? 8 : (SEQ > 1 ? 4 : 1);
   static constexpr size_t buf = static_cast<size_t>(G) * lds_padded_size(N) * csz;
   static constexpr size_t edge = static_cast<size_t>(G) * 2 * kEdge * csz;
   static constexpr size_t red = 2 * (NT / 64) * sizeof(double);
@@ -165,221 +192,268 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
   constexpr int MB0 = A0::MB, M0 = A0::M, RH = R0 / 2;
 
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tid = threadIdx.x;
-  const int g = tid / TPT, u = tid - g * TPT, lane = tid & 63, wg = u >> 6;
-  C* const buf0 = reinterpret_cast<C*>(smem);
-  C* const edge0 = reinterpret_cast<C*>(smem + Lay::buf);
-  double* const red = reinterpret_cast<double*>(smem + Lay::buf + Lay::edge);
-  Acc* const chunk0 = reinterpret_cast<Acc*>(smem + Lay::buf + Lay::edge + Lay::red);
-  C* const buf = buf0 + g * lds_padded_size(N);
-  C* const edge = edge0 + g * 2 * kEdge;
-  Acc* const chunk = chunk0 + g * WPT;
 
   // Workgroups are dealt round-robin to the 8 XCDs: each XCD gets a
-  // contiguous range of q so neighbouring output columns share its L2.
-  const int qbase = ((blockIdx.x & 7) * wg_per_xcd + (blockIdx.x >> 3)) * G;
-  const int q = qbase + g;
-  const bool active = q < nq;
+  // contiguous range of q (of output columns when SEQ > 1) so neighbouring
+  // output columns share its L2.
+  constexpr int SEQ = Lay::SEQ;
+  const int wgi = (blockIdx.x & 7) * wg_per_xcd + (blockIdx.x >> 3);
+  if (SEQ > 1 && SEQ * wgi >= nq) return;  // whole workgroup: before any barrier
+  const int qbase = wgi * G;
   const int L = a.L;
-  const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(active ? q : 0) * a.Lp;
   const C* __restrict__ tw = static_cast<const C*>(a.twt);
+  OutT* __restrict__ out = static_cast<OutT*>(a.out);
+  const size_t nqs = static_cast<size_t>(a.ny) * a.nxh;
+  const Acc inv_n = static_cast<Acc>(1) / static_cast<Acc>(N);
+  Acc dsave[SEQ > 1 ? Lay::KMAX : 1];  // SEQ > 1: D of the first column per lag iteration
 
-  // 1. first-pass inputs straight from HBM (clamped indices, no branches;
-  // samples t >= L are replaced by the zero padding afterwards)
-  C v0[MB0][R0];
+#pragma nounroll
+  for (int sq = 0; sq < SEQ; ++sq) {
+    // thread-derived addresses are rebuilt per column rather than kept live
+    // across the loop (the thread index is opaque to the optimiser there)
+    int tid = threadIdx.x;
+    if constexpr (SEQ > 1) asm volatile("" : "+v"(tid));
+    const int g = tid / TPT, u = tid - g * TPT, lane = tid & 63, wg = u >> 6;
+    C* const buf0 = reinterpret_cast<C*>(smem);
+    C* const edge0 = reinterpret_cast<C*>(smem + Lay::buf);
+    double* const red = reinterpret_cast<double*>(smem + Lay::buf + Lay::edge);
+    Acc* const chunk0 = reinterpret_cast<Acc*>(smem + Lay::buf + Lay::edge + Lay::red);
+    C* const buf = buf0 + g * lds_padded_size(N);
+    C* const edge = edge0 + g * 2 * kEdge;
+    Acc* const chunk = chunk0 + g * WPT;
+    int q;
+    if constexpr (SEQ > 1) {
+      const int o = SEQ * wgi + sq;  // output column
+      q = o < nq ? q_of_column(o, a.ny, a.nxh) : nq;
+    } else {
+      q = qbase + g;
+    }
+    const bool active = q < nq;
+    const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(active ? q : 0) * a.Lp;
+
+    // 1. first-pass inputs straight from HBM (clamped indices, no branches;
+    // samples t >= L are replaced by the zero padding afterwards)
+    C v0[MB0][R0];
 #pragma unroll
-  for (int b = 0; b < MB0; ++b)
+    for (int b = 0; b < MB0; ++b)
 #pragma unroll
-    for (int r = 0; r < R0; ++r) {
-      const int t = u + b * TPT + r * M0;
-      if (r < RH) {
-        const C f = src[min(t, L - 1)];
-        v0[b][r] = (active && t < L) ? f : C{0, 0};
-      } else {
-        v0[b][r] = C{0, 0};
+      for (int r = 0; r < R0; ++r) {
+        const int t = u + b * TPT + r * M0;
+        if (r < RH) {
+          const C f = src[min(t, L - 1)];
+          v0[b][r] = (active && t < L) ? f : C{0, 0};
+        } else {
+          v0[b][r] = C{0, 0};
+        }
       }
-    }
 
-  // 2. temporal mean (the compensated paths transform F - mean)
-  double sr = 0.0, si = 0.0;
-#pragma unroll
-  for (int b = 0; b < MB0; ++b)
-#pragma unroll
-    for (int r = 0; r < RH; ++r) {
-      sr += static_cast<double>(v0[b][r].x);
-      si += static_cast<double>(v0[b][r].y);
-    }
-  group_sum2<WPT>(sr, si, red);
-  const double rl = recip<double>(L);
-  const double mr = sr * rl, mi = si * rl;
-  const T m_r = SUB ? static_cast<T>(mr) : T(0), m_i = SUB ? static_cast<T>(mi) : T(0);
-  if constexpr (SUB) {
+    // 2. temporal mean (the compensated paths transform F - mean)
+    double sr = 0.0, si = 0.0;
 #pragma unroll
     for (int b = 0; b < MB0; ++b)
 #pragma unroll
       for (int r = 0; r < RH; ++r) {
-        const int t = u + b * TPT + r * M0;
-        if (t < L) {
-          v0[b][r].x -= m_r;
-          v0[b][r].y -= m_i;
-        }
+        sr += static_cast<double>(v0[b][r].x);
+        si += static_cast<double>(v0[b][r].y);
       }
-  }
-  // sum of the (subtracted) series, for VAR and PS
-  const double fsr = sr - static_cast<double>(L) * static_cast<double>(m_r);
-  const double fsi = si - static_cast<double>(L) * static_cast<double>(m_i);
-
-  // 3. first transform G = FFT(F): first pass on the registers, last pass
-  // back into registers; P = |G|^2 goes to LDS as z(n) = P(2n) + i P(2n+1)
-  A0::compute_regs(v0);
-  A0::store(buf, v0, u);
-  __syncthreads();
-  p2_passes<T, LOGN, false, 1, NP - 1>(buf, tw, u);
-  {
-    C v2[AL::MB][RL];
-    AL::compute(buf, v2, tw + TWA, u);
-    __syncthreads();
-    T* const zf = reinterpret_cast<T*>(buf);
-#pragma unroll
-    for (int b = 0; b < AL::MB; ++b)
-#pragma unroll
-      for (int r = 0; r < RL; ++r) {
-        const int w = u + b * TPT + r * (N / RL);  // output position of the last pass
-        zf[2 * padi(w >> 1) + (w & 1)] = v2[b][r].x * v2[b][r].x + v2[b][r].y * v2[b][r].y;
-      }
-  }
-  __syncthreads();
-  // 4-5. prefix sums psum(t + 1) of |F(t) - mean|^2 from a second
-  // (cache-resident) read of the series into the buffer half the second
-  // transform leaves free, first and last kEdge samples into `edge`; and the
-  // second transform at half length, Z = FFT_{N/2}(z).  PP (the paired
-  // N = 4096 plan): the transform alternates between the halves (one barrier
-  // per pass) and the sums follow it into the half it leaves; otherwise the
-  // sums come first, so the re-read follows the first read closely, and the
-  // transform runs in place in the first half.  Both orders were measured.
-  constexpr bool PP = Lay::kPair;
-  constexpr int NPH = p2_half_plan(LOGN, F64).np;
-  constexpr bool ZA = !PP || NPH % 2 == 0;  // Z ends in the first half
-  auto prefix_sums = [&](Acc* const tail) {
-    // each thread: a contiguous segment of seg <= SEGMAX samples, local running
-    // sums, then one exclusive scan over the group's threads; seg is odd, so
-    // the 8-byte stores of neighbouring threads fall in different LDS banks
-    constexpr int SEGMAX = ((N / 2 + TPT - 1) / TPT) | 1;
-    const int seg = ((L + TPT - 1) / TPT) | 1;
-    const int ts = u * seg;
-    Acc pre[SEGMAX];
-    Acc run = 0;
-#pragma unroll
-    for (int i = 0; i < SEGMAX; ++i) {
-      const int t = ts + i;
-      Acc val = 0;
-      if (i < seg && active && t < L) {
-        C f = src[t];
-        f.x -= m_r;
-        f.y -= m_i;
-        if constexpr (SUB) {
-          if (t < kEdge) edge[t] = f;
-          if (t >= L - kEdge) edge[t - L + 2 * kEdge] = f;
-        }
-        val = static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
-      }
-      run += val;
-      pre[i] = run;
-    }
-    const Acc inc = wave::incl_scan(run);
-    Acc exc = wave::from_prev_lane(inc);  // 0 in lane 0
-    if (lane == 63) chunk[wg] = inc;
-    __syncthreads();
-#pragma unroll
-    for (int w = 0; w < WPT; ++w)
-      if (w < wg) exc += chunk[w];
-#pragma unroll
-    for (int i = 0; i < SEGMAX; ++i) {
-      const int t = ts + i;
-      if (i < seg && t < L - 1) tail[t] = pre[i] + exc;
-    }
-    __syncthreads();
-  };
-  Acc* const tail = reinterpret_cast<Acc*>(ZA ? buf + padi(NH) : buf);
-  if constexpr (PP) {
-    p2_half_pp<T, LOGN, 0, NPH>(buf, buf + padi(NH), tw, u);
-    prefix_sums(tail);
-  } else {
-    prefix_sums(tail);
-    p2_passes<T, LOGN, true, 0, NPH>(buf, tw, u);
-  }
-
-  // 6. lags: thread tid serves q group h = tid % G and lag indices
-  // nl - 1 - tid / G, stepping by NT / G, so neighbouring lanes store the same
-  // lag of neighbouring q.  Lags in reverse order: the largest lags (direct
-  // sums, longest loops) all fall in the first wave(s).
-  OutT* __restrict__ out = static_cast<OutT*>(a.out);
-  const size_t nqs = static_cast<size_t>(a.ny) * a.nxh;
-  const Acc inv_n = static_cast<Acc>(1) / static_cast<Acc>(N);
-  {
-    const int h = tid % G, v = tid / G, qh = qbase + h;
-    if (qh < nq) {
-      const C* const hgrp = buf0 + h * lds_padded_size(N);
-      const C* const hbuf = ZA ? hgrp : hgrp + padi(NH);  // Z
-      const Acc* const htail = reinterpret_cast<const Acc*>(ZA ? hgrp + padi(NH) : hgrp);
-      const C* const hedge = edge0 + h * 2 * kEdge;
-      Acc hptot = 0;
-#pragma unroll
-      for (int w = 0; w < WPT; ++w) hptot += chunk0[h * WPT + w];
-      const int ky = qh / a.nxh, kx = qh - ky * a.nxh;
-      const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
-      for (int i = a.nl - 1 - v; i >= 0; i -= Lay::NT / G) {
-        const int d = static_cast<int>(a.lags[i]);
-        Acc D;
-        if (SUB && L - d <= kEdge) {
-          Acc acc = 0;
-          for (int t = 0; t < L - d; ++t) {
-            const int s = t + d;
-            const C x = s < kEdge ? hedge[s] : hedge[s - L + 2 * kEdge];
-            const C y = hedge[t];
-            const Acc dr = static_cast<Acc>(x.x) - static_cast<Acc>(y.x);
-            const Acc di = static_cast<Acc>(x.y) - static_cast<Acc>(y.y);
-            acc += dr * dr + di * di;
-          }
-          D = acc * recip<Acc>(L - d);
-        } else {
-          const Acc head = htail[L - d - 1];   // psum(L - d)
-          const Acc tl = hptot - htail[d - 1];  // psum(L) - psum(d)
-          // N C(d) = Re FFT_N(P)(d) from A = Z(d), B = Z(N/2 - d) and
-          // (c, s) = (cos, sin)(2 pi d / N): 2 Re X = A.x + B.x + c (A.y + B.y) - s (A.x - B.x)
-          const C za = hbuf[padi(d)], zb = hbuf[padi((NH - d) & (NH - 1))], cs = tw[TWP + d];
-          const Acc ax = za.x, ay = za.y, bx = zb.x, by = zb.y;
-          const Acc re = (ax + bx) + static_cast<Acc>(cs.x) * (ay + by) - static_cast<Acc>(cs.y) * (ax - bx);
-          const Acc c = re * (Acc(0.5) * inv_n);
-          D = (head + tl - 2 * c) * recip<Acc>(L - d);
-        }
-        out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(D);
-      }
-    }
-  }
-  if (active && u == 0) {
-    Acc ptot = 0;
-#pragma unroll
-    for (int w = 0; w < WPT; ++w) ptot += chunk[w];
-    const int ky = q / a.nxh, kx = q - ky * a.nxh;
-    const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
-    const Acc mr2 = static_cast<Acc>(fsr / L), mi2 = static_cast<Acc>(fsi / L);
-    Acc ps, var;
+    group_sum2<WPT>(sr, si, red);
+    const double rl = recip<double>(L);
+    const double mr = sr * rl, mi = si * rl;
+    const T m_r = SUB ? static_cast<T>(mr) : T(0), m_i = SUB ? static_cast<T>(mi) : T(0);
     if constexpr (SUB) {
-      var = ptot / L - (mr2 * mr2 + mi2 * mi2);
-      Acc fr = static_cast<Acc>(m_r) + mr2, fi = static_cast<Acc>(m_i) + mi2;
-      if (a.S0) {
-        const C z = static_cast<const C*>(a.S0)[static_cast<size_t>(q) * a.s0_stride];
-        fr += static_cast<Acc>(z.x);
-        fi += static_cast<Acc>(z.y);
-      }
-      ps = var + fr * fr + fi * fi;
-    } else {
-      ps = ptot / L;
-      var = ps - (mr2 * mr2 + mi2 * mi2);
+#pragma unroll
+      for (int b = 0; b < MB0; ++b)
+#pragma unroll
+        for (int r = 0; r < RH; ++r) {
+          const int t = u + b * TPT + r * M0;
+          if (t < L) {
+            v0[b][r].x -= m_r;
+            v0[b][r].y -= m_i;
+          }
+        }
     }
-    out[static_cast<size_t>(a.nl) * nqs + oq] = static_cast<OutT>(ps);
-    out[static_cast<size_t>(a.nl + 1) * nqs + oq] = static_cast<OutT>(var);
+    // sum of the (subtracted) series, for VAR and PS
+    const double fsr = sr - static_cast<double>(L) * static_cast<double>(m_r);
+    const double fsi = si - static_cast<double>(L) * static_cast<double>(m_i);
+
+    // 3. first transform G = FFT(F): first pass on the registers, last pass
+    // back into registers; P = |G|^2 goes to LDS as z(n) = P(2n) + i P(2n+1)
+    A0::compute_regs(v0);
+    A0::store(buf, v0, u);
+    __syncthreads();
+    p2_passes<T, LOGN, false, 1, NP - 1>(buf, tw, u);
+    {
+      C v2[AL::MB][RL];
+      AL::compute(buf, v2, tw + TWA, u);
+      __syncthreads();
+      T* const zf = reinterpret_cast<T*>(buf);
+#pragma unroll
+      for (int b = 0; b < AL::MB; ++b)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const int w = u + b * TPT + r * (N / RL);  // output position of the last pass
+          zf[2 * padi(w >> 1) + (w & 1)] = v2[b][r].x * v2[b][r].x + v2[b][r].y * v2[b][r].y;
+        }
+    }
+    __syncthreads();
+    // 4-5. prefix sums psum(t + 1) of |F(t) - mean|^2 from a second
+    // (cache-resident) read of the series into the buffer half the second
+    // transform leaves free, first and last kEdge samples into `edge`; and the
+    // second transform at half length, Z = FFT_{N/2}(z).  PP (the paired
+    // N = 4096 plan): the transform alternates between the halves (one barrier
+    // per pass) and the sums follow it into the half it leaves; otherwise the
+    // sums come first, so the re-read follows the first read closely, and the
+    // transform runs in place in the first half.  Both orders were measured.
+    constexpr bool PP = Lay::kPair;
+    constexpr int NPH = p2_half_plan(LOGN, F64).np;
+    constexpr bool ZA = !PP || NPH % 2 == 0;  // Z ends in the first half
+    auto prefix_sums = [&](Acc* const tail) {
+      // each thread: a contiguous segment of seg <= SEGMAX samples, local running
+      // sums, then one exclusive scan over the group's threads; seg is odd, so
+      // the 8-byte stores of neighbouring threads fall in different LDS banks
+      constexpr int SEGMAX = ((N / 2 + TPT - 1) / TPT) | 1;
+      const int seg = ((L + TPT - 1) / TPT) | 1;
+      const int ts = u * seg;
+      Acc pre[SEGMAX];
+      Acc run = 0;
+#pragma unroll
+      for (int i = 0; i < SEGMAX; ++i) {
+        const int t = ts + i;
+        Acc val = 0;
+        if (i < seg && active && t < L) {
+          C f = src[t];
+          f.x -= m_r;
+          f.y -= m_i;
+          if constexpr (SUB) {
+            if (t < kEdge) edge[t] = f;
+            if (t >= L - kEdge) edge[t - L + 2 * kEdge] = f;
+          }
+          val = static_cast<Acc>(f.x) * static_cast<Acc>(f.x) + static_cast<Acc>(f.y) * static_cast<Acc>(f.y);
+        }
+        run += val;
+        pre[i] = run;
+      }
+      const Acc inc = wave::incl_scan(run);
+      Acc exc = wave::from_prev_lane(inc);  // 0 in lane 0
+      if (lane == 63) chunk[wg] = inc;
+      __syncthreads();
+#pragma unroll
+      for (int w = 0; w < WPT; ++w)
+        if (w < wg) exc += chunk[w];
+#pragma unroll
+      for (int i = 0; i < SEGMAX; ++i) {
+        const int t = ts + i;
+        if (i < seg && t < L - 1) tail[t] = pre[i] + exc;
+      }
+      __syncthreads();
+    };
+    Acc* const tail = reinterpret_cast<Acc*>(ZA ? buf + padi(NH) : buf);
+    if constexpr (PP) {
+      p2_half_pp<T, LOGN, 0, NPH>(buf, buf + padi(NH), tw, u);
+      prefix_sums(tail);
+    } else {
+      prefix_sums(tail);
+      p2_passes<T, LOGN, true, 0, NPH>(buf, tw, u);
+    }
+
+    // 6. lags: thread tid serves q group h = tid % G and lag indices
+    // nl - 1 - tid / G, stepping by NT / G, so neighbouring lanes store the same
+    // lag of neighbouring q.  Lags in reverse order: the largest lags (direct
+    // sums, longest loops) all fall in the first wave(s).
+    {
+      const int h = tid % G, v = tid / G, qh = SEQ > 1 ? q : qbase + h;
+      if (SEQ > 1 || qh < nq) {
+        const C* const hgrp = buf0 + h * lds_padded_size(N);
+        const C* const hbuf = ZA ? hgrp : hgrp + padi(NH);  // Z
+        const Acc* const htail = reinterpret_cast<const Acc*>(ZA ? hgrp + padi(NH) : hgrp);
+        const C* const hedge = edge0 + h * 2 * kEdge;
+        Acc hptot = 0;
+#pragma unroll
+        for (int w = 0; w < WPT; ++w) hptot += chunk0[h * WPT + w];
+        auto lag_value = [&](int i) -> Acc {
+          const int d = static_cast<int>(a.lags[i]);
+          Acc D;
+          if (SUB && L - d <= kEdge) {
+            Acc acc = 0;
+            for (int t = 0; t < L - d; ++t) {
+              const int e = t + d;
+              const C x = e < kEdge ? hedge[e] : hedge[e - L + 2 * kEdge];
+              const C y = hedge[t];
+              const Acc dr = static_cast<Acc>(x.x) - static_cast<Acc>(y.x);
+              const Acc di = static_cast<Acc>(x.y) - static_cast<Acc>(y.y);
+              acc += dr * dr + di * di;
+            }
+            D = acc * recip<Acc>(L - d);
+          } else {
+            const Acc head = htail[L - d - 1];   // psum(L - d)
+            const Acc tl = hptot - htail[d - 1];  // psum(L) - psum(d)
+            // N C(d) = Re FFT_N(P)(d) from A = Z(d), B = Z(N/2 - d) and
+            // (c, s) = (cos, sin)(2 pi d / N): 2 Re X = A.x + B.x + c (A.y + B.y) - s (A.x - B.x)
+            const C za = hbuf[padi(d)], zb = hbuf[padi((NH - d) & (NH - 1))], cs = tw[TWP + d];
+            const Acc ax = za.x, ay = za.y, bx = zb.x, by = zb.y;
+            const Acc re = (ax + bx) + static_cast<Acc>(cs.x) * (ay + by) - static_cast<Acc>(cs.y) * (ax - bx);
+            const Acc c = re * (Acc(0.5) * inv_n);
+            D = (head + tl - 2 * c) * recip<Acc>(L - d);
+          }
+          return D;
+        };
+        if constexpr (SEQ > 1) {
+          // lag iteration k: index nl - 1 - tid - k NT; the second column stores
+          // both values (one access where the pair is aligned: the first column
+          // is even, so that needs an even row length and an aligned base)
+          const size_t o0 = static_cast<size_t>(SEQ) * wgi;
+          const bool vec = (nqs & 1) == 0 && reinterpret_cast<uintptr_t>(out) % sizeof(OutPair<OutT>) == 0;
+#pragma unroll
+          for (int k = 0; k < Lay::KMAX; ++k) {
+            const int i = a.nl - 1 - v - k * Lay::NT;
+            if (i < 0) break;
+            const Acc D = lag_value(i);
+            if (sq == 0) {
+              dsave[k] = D;
+            } else {
+              OutT* const dst = out + static_cast<size_t>(i) * nqs + o0;
+              if (active && vec) {
+                *reinterpret_cast<OutPair<OutT>*>(dst) =
+                    OutPair<OutT>{static_cast<OutT>(dsave[k]), static_cast<OutT>(D)};
+              } else {
+                dst[0] = static_cast<OutT>(dsave[k]);
+                if (active) dst[1] = static_cast<OutT>(D);
+              }
+            }
+          }
+        } else {
+          const int ky = qh / a.nxh, kx = qh - ky * a.nxh;
+          const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
+          for (int i = a.nl - 1 - v; i >= 0; i -= Lay::NT / G)
+            out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(lag_value(i));
+        }
+      }
+    }
+    if (active && u == 0) {
+      Acc ptot = 0;
+#pragma unroll
+      for (int w = 0; w < WPT; ++w) ptot += chunk[w];
+      const int ky = q / a.nxh, kx = q - ky * a.nxh;
+      const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
+      const Acc mr2 = static_cast<Acc>(fsr / L), mi2 = static_cast<Acc>(fsi / L);
+      Acc ps, var;
+      if constexpr (SUB) {
+        var = ptot / L - (mr2 * mr2 + mi2 * mi2);
+        Acc fr = static_cast<Acc>(m_r) + mr2, fi = static_cast<Acc>(m_i) + mi2;
+        if (a.S0) {
+          const C z = static_cast<const C*>(a.S0)[static_cast<size_t>(q) * a.s0_stride];
+          fr += static_cast<Acc>(z.x);
+          fi += static_cast<Acc>(z.y);
+        }
+        ps = var + fr * fr + fi * fi;
+      } else {
+        ps = ptot / L;
+        var = ps - (mr2 * mr2 + mi2 * mi2);
+      }
+      out[static_cast<size_t>(a.nl) * nqs + oq] = static_cast<OutT>(ps);
+      out[static_cast<size_t>(a.nl + 1) * nqs + oq] = static_cast<OutT>(var);
+    }
+    if (SEQ > 1 && sq + 1 < SEQ) __syncthreads();  // the next column reuses the LDS
   }
 }
 
@@ -387,7 +461,9 @@ template <typename T, typename OutT, int LOGN, bool SUB>
 hipError_t launch_one(const TimeArgs& a, hipStream_t s) {
   using Lay = P2Lds<LOGN, T, SUB>;
   const int nq = a.ny * a.nxh;
-  const int wgs = (nq + Lay::G - 1) / Lay::G;
+  // two-column plans keep at most KMAX lags per thread in registers (nl < L <= N/2 fits)
+  if (Lay::SEQ > 1 && a.nl > Lay::KMAX * Lay::NT) return hipErrorInvalidValue;
+  const int wgs = (nq + Lay::G * Lay::SEQ - 1) / (Lay::G * Lay::SEQ);
   const int wpx = (wgs + 7) / 8;
   const size_t lds = Lay::bytes;
   auto kernel = k_time_p2<T, OutT, LOGN, SUB>;

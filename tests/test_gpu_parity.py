@@ -178,11 +178,14 @@ def test_every_time_plan_matches_oracle(hip, L, N, precision):
     CHECK[precision](out, ref)
 
 
-@pytest.mark.parametrize("L", [700, 1500])
+@pytest.mark.parametrize("L", [700, 1500, 2500, 4500])
 def test_time_plan_odd_number_of_q(hip, L):
     """45 q-series (9 x 8 images): the plans that put several q in one
     workgroup (two at N = 4096, fp32c) end with a partly empty workgroup
-    whose idle groups still serve the lag loop of the others."""
+    whose idle groups still serve the lag loop of the others; the plans that
+    run two output columns one after the other (N = 8192 and 16384, fp32c)
+    end with a lone column, and with 45 values per lag row the column pairs
+    are stored as two values rather than one aligned pair."""
     img = ar1_speckle(L, 9, 8, sigma=1.1, diffusivity=0.08, mean=700, std=110, seed=L + 7)
     lags = np.arange(1, L)
     ref = onp.ddm_fft(img, lags, 8, 9, 2 * L, None, subtract_mean=True)
@@ -213,11 +216,14 @@ def test_every_input_dtype(hip, dtype):
     check_fp32c(hip.ddm_fft(img, lags, 18, 16, 24, None, precision="fp32c"), ref)
 
 
-def test_float32_output_dtype(hip):
-    img = ar1_speckle(20, 16, 16, seed=9)
-    lags = np.arange(1, 20)
-    ref = onp.ddm_fft(img, lags, 16, 16, 40, subtract_mean=True)
-    out = hip.ddm_fft(img, lags, 16, 16, 40, None, precision="fp32c", out_dtype=np.float32)
+@pytest.mark.parametrize("L", [20, 2500])
+def test_float32_output_dtype(hip, L):
+    """Also at N = 8192, whose column pairs are stored as one 8-byte pair."""
+    img = ar1_speckle(L, 16, 16, seed=9)
+    lags = np.arange(1, L)
+    ref = onp.ddm_fft(img, lags, 16, 16, 2 * L, subtract_mean=True)
+    out = hip.ddm_fft(img, lags, 16, 16, next_fast_len(2 * L, "hip"), None, precision="fp32c",
+                      out_dtype=np.float32)
     assert out.dtype == np.float32
     assert np.isclose(out, ref, rtol=1e-3, atol=0).all()
 
