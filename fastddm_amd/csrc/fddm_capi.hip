@@ -333,7 +333,7 @@ int fddm::plan_create_ex(fddm_plan** out, int mode, int dtype, uint64_t L, uint6
   // Power-of-two spatial kernels where the size and frame block allow them.
   {
     const int lx = ilog2_exact(nx), ly = ilog2_exact(ny);
-    if (p2_spatial_ok(lx) && (p->Tb == 8 || p->Tb == 16) &&
+    if (p2_spatial_ok(lx) && (p->Tb == 4 || p->Tb == 8 || p->Tb == 16) &&
         lds_padded_size((p->Tb / 2) * static_cast<int>(nx)) * csz <= 96 * 1024)
       p->logn_x = lx;
     if (p2_spatial_ok(ly) && p->Tb >= 4) p->logn_y = ly;

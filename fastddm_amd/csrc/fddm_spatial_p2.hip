@@ -341,6 +341,7 @@ template <int LOGX, typename Tin, typename T>
 hipError_t rows_one(const RowArgs& a, unsigned grid, hipStream_t s) {
   if (a.Tb == 16) return rows_np<LOGX, Tin, T, 8>(a, grid, s);
   if (a.Tb == 8) return rows_np<LOGX, Tin, T, 4>(a, grid, s);
+  if (a.Tb == 4) return rows_np<LOGX, Tin, T, 2>(a, grid, s);
   return hipErrorInvalidValue;
 }
 

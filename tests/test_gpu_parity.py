@@ -81,6 +81,9 @@ def test_fft_engine_matches_numpy(hip, precision, tol):
     ((5, 255, 509), 512, 256, True),
     ((4, 512, 1024), 1024, 512, False),
     ((3, 1024, 256), 256, 1024, True),
+    # 4-frame blocks (fp64 from ny = 1024, fp32 from ny = 2048): two pair rows per workgroup
+    ((3, 1024, 1024), 1024, 1024, True),
+    ((3, 2048, 256), 256, 2048, False),
 ])
 def test_spatial_spectra_match_oracle(hip, precision, shape, nx, ny, window):
     L, H, W = shape
