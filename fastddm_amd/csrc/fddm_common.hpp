@@ -43,12 +43,9 @@ struct FftDesc {
 // every 32 (see lds_pad in fddm_fft.hpp).
 __host__ __device__ constexpr int lds_padded_size(int n) { return n + (n >> 5) + 1; }
 
-// Power-of-two plans of the time kernel: N = 2^logn points, T threads per
-// transform, three passes of radices R0, R1, R2 (forward order; the second
-// transform runs them in reverse).  Every pass has a whole number of
-// butterflies per thread: N / (R * T) >= 1.
 // Time-axis plans for N = 2^logn: threads per transform, number of passes
-// (3 or 4) and the radices of the first transform in order.  For
+// (3 or 4) and the radices of the first transform in order; the first and
+// last passes have a whole number of butterflies per thread.  For
 // FP32 at N >= 4096 the transform is spread over 512 or 1024 threads (radix
 // 8 or 16 passes) so that a CU holds 32 (N = 4096) or 16 waves despite the
 // LDS buffer; FP64 keeps three passes on 256 threads, which measured faster

@@ -29,7 +29,7 @@
 //
 // The output is lag-major, so each lag store of one q writes one value into
 // its own line.  Workgroups of several q interleave them across lanes; the
-// compensated FP32 plans from N = 8192, whose LDS holds a single q, instead
+// compensated plans from N = 8192, whose LDS holds a single q, instead
 // run steps 1-6 for two neighbouring output columns in turn, keep the first
 // column's D in registers and store each lag's two values together.
 #include <type_traits>
@@ -116,15 +116,16 @@ struct P2Lds {
   static constexpr int NT = TPT < kMinThreads ? kMinThreads : (kPair ? 2 * TPT : TPT);  // workgroup size
   static constexpr int G = NT / TPT;
   // paired workgroups: two per CU hold 8 waves per SIMD when at most 64 VGPRs are used
-  // Compensated FP32 plans from N = 8192 hold one q per workgroup (the LDS
-  // has no room for a second) and leave registers free: the workgroup
-  // transforms the q of two neighbouring output columns one after the other
-  // and keeps the first one's D in registers (KMAX lag iterations per
-  // thread), so each lag store writes both columns (16 bytes) at once.
-  static constexpr int SEQ = (SUB && sizeof(T) == 4 && !kPair && TPT >= kMinThreads && LOGN >= 13) ? 2 : 1;
+  // Compensated plans from N = 8192 hold one q per workgroup (the LDS has no
+  // room for a second) and leave registers free: the workgroup transforms
+  // the q of two neighbouring output columns one after the other and keeps
+  // the first one's D in registers (KMAX lag iterations per thread), so each
+  // lag store writes both columns (16 bytes) at once.
+  static constexpr int SEQ = (SUB && !kPair && TPT >= kMinThreads && LOGN >= 13) ? 2 : 1;
   static constexpr int KMAX = (N / 2 + NT - 1) / NT;
-  // registers for the 4 waves per SIMD the LDS allows
-  static constexpr int kMinWaves = kPair ? 8 : (SEQ > 1 ? 4 : 1);
+  // registers for the 4 waves per SIMD the LDS allows (FP32; FP64 at N = 8192
+  // has one workgroup of 4 waves per CU)
+  static constexpr int kMinWaves = kPair ? 8 : (SEQ > 1 && sizeof(T) == 4 ? 4 : 1);
   static constexpr size_t buf = static_cast<size_t>(G) * lds_padded_size(N) * csz;
   static constexpr size_t edge = static_cast<size_t>(G) * 2 * kEdge * csz;
   static constexpr size_t red = 2 * (NT / 64) * sizeof(double);
