@@ -255,5 +255,42 @@ __device__ __forceinline__ void fft3_lds(cplx_t<T>* buf, const cplx_t<T>* __rest
   }
 }
 
+// LDS writes of this wave complete and visible to its own later reads, and
+// no memory access moved across this point by the compiler.  (Each pass of
+// fft3_wave reads all its inputs into registers before any lane writes, so
+// in-place passes need this only between passes.)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0), other counters left alone
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Whole forward transform of one length-N row in LDS by one wave (lane in
+// [0, 64)), three passes, no workgroup barriers: the row's LDS region is
+// this wave's own.
+template <typename T, int N, int R0, int R1, int R2>
+__device__ __forceinline__ void fft3_wave(cplx_t<T>* row, const cplx_t<T>* __restrict__ tw, int lane) {
+  using C = cplx_t<T>;
+  using A0 = Pass<T, N, R0, 1, 64>;
+  using A1 = Pass<T, N, R1, R0, 64>;
+  using A2 = Pass<T, N, R2, R0 * R1, 64>;
+  {
+    C v[A0::MB][R0];
+    A0::compute(row, v, tw, lane);
+    A0::store(row, v, lane);
+    wave_lds_sync();
+  }
+  {
+    C v[A1::MB][R1];
+    A1::compute(row, v, tw, lane);
+    A1::store(row, v, lane);
+    wave_lds_sync();
+  }
+  {
+    C v[A2::MB][R2];
+    A2::compute(row, v, tw + R0 * R1, lane);
+    A2::store(row, v, lane);
+  }
+}
+
 }  // namespace p2
 }  // namespace fddm

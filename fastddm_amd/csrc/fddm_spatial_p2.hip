@@ -138,23 +138,29 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems
 }
 
 // Full-width rows of 2-byte pixels (W == NX <= 512), FP32: a persistent form
-// of k_rows_p2.  Workgroup b takes items b, b + grid, ... (item = tb * H + y);
-// the next item's pixels and frame-0 row are loaded into registers before
-// the current item's FFT and stores, so their latency overlaps the transform
-// instead of opening each item.
+// of k_rows_p2 with one wave per pair row.  Workgroup b takes items b,
+// b + grid, ... (item = tb * H + y); wave p loads, converts and transforms
+// pair row p in its own LDS region with no workgroup barrier, loading the
+// next item's pixels and frame-0 row into registers before the transform so
+// their latency overlaps it.  Two barriers per item remain: before the split
+// into half spectra (which reads every pair row) and before the next item
+// rewrites the rows.
 template <typename Tin, typename T, int LOGX, int NP>
-__global__ void __launch_bounds__(kThreads, 8) k_rows_p2_pf(RowArgs a, uint32_t nitems) {
+__global__ void __launch_bounds__(64 * NP, 8) k_rows_p2_pf(RowArgs a, uint32_t nitems) {
   using C = cplx_t<T>;
+  constexpr int NT = 64 * NP;
   constexpr int NX = 1 << LOGX, NXH = NX / 2 + 1, TB = 2 * NP;
   constexpr SpPlan P = sp_plan(LOGX);
   constexpr int VX = 8 / sizeof(Tin), GV = NX / VX;
-  constexpr int PER = (NP * GV + kThreads - 1) / kThreads;  // 8-byte vectors per thread and frame row
+  constexpr int PER = GV / 64;  // 8-byte vectors per lane and frame row
+  static_assert(GV % 64 == 0, "whole vectors per lane");
   struct alignas(8) Vec {
     Tin v[VX];
   };
   extern __shared__ __align__(16) unsigned char smem[];
   C* const buf = reinterpret_cast<C*>(smem);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, p = tid >> 6;  // p: this wave's pair row
+  C* const row = buf + padi(p * NX);  // p NX is a multiple of 32: padded offsets add
   const int H = a.H;
   const Tin* __restrict__ img = static_cast<const Tin*>(a.img);
   const T* __restrict__ win = static_cast<const T*>(a.window);
@@ -167,14 +173,11 @@ __global__ void __launch_bounds__(kThreads, 8) k_rows_p2_pf(RowArgs a, uint32_t 
   Vec va[PER], vb[PER], v0[PER];
   auto load = [&](uint32_t item) {
     const int y = static_cast<int>(item % H);
-    const uint64_t t0 = static_cast<uint64_t>(item / H) * TB;
+    const uint64_t ta = static_cast<uint64_t>(item / H) * TB + 2 * p;
     const Tin* __restrict__ row0 = img + static_cast<size_t>(y) * NX;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int e = tid + k * kThreads;
-      if ((NP * GV) % kThreads != 0 && e >= NP * GV) break;
-      const int p = e / GV, x0 = (e - p * GV) * VX;
-      const uint64_t ta = t0 + 2 * p;
+      const int x0 = (lane + 64 * k) * VX;
       va[k] = ta < a.frames ? *reinterpret_cast<const Vec*>(row0 + ta * plane + x0) : Vec{};
       vb[k] = ta + 1 < a.frames ? *reinterpret_cast<const Vec*>(row0 + (ta + 1) * plane + x0) : Vec{};
       v0[k] = sub ? *reinterpret_cast<const Vec*>(row0 + x0) : Vec{};
@@ -185,16 +188,12 @@ __global__ void __launch_bounds__(kThreads, 8) k_rows_p2_pf(RowArgs a, uint32_t 
   if (item < nitems) load(item);
   for (; item < nitems; item += gridDim.x) {
     const int y = static_cast<int>(item % H);
-    const uint64_t tb = item / H, t0 = tb * TB;
-    // registers -> LDS: pairs (2p, 2p+1) as one complex row, element (p, x)
-    // at logical p NX + x
+    const uint64_t tb = item / H, ta = tb * TB + 2 * p;
+    // registers -> this wave's row: frames (2p, 2p+1) as one complex row
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int e = tid + k * kThreads;
-      if ((NP * GV) % kThreads != 0 && e >= NP * GV) break;
-      const int p = e / GV, x0 = (e - p * GV) * VX;
-      const uint64_t ta = t0 + 2 * p;
-      C* d = buf + padi(p * NX + x0);  // VX divides 32: the VX slots are contiguous
+      const int x0 = (lane + 64 * k) * VX;
+      C* d = row + padi(x0);  // VX divides 32: the VX slots are contiguous
 #pragma unroll
       for (int v = 0; v < VX; ++v) {
         const T w = win ? win[static_cast<size_t>(y) * NX + x0 + v] : T(1);
@@ -203,17 +202,18 @@ __global__ void __launch_bounds__(kThreads, 8) k_rows_p2_pf(RowArgs a, uint32_t 
         d[v] = C{fa, fb};
       }
     }
-    __syncthreads();
+    wave_lds_sync();
     if (item + gridDim.x < nitems) load(item + gridDim.x);
-    fft3_lds<T, NX, P.r0, P.r1, P.r2, kThreads, NP, false, 1, NX>(buf, tw, tid);
+    fft3_wave<T, NX, P.r0, P.r1, P.r2>(row, tw, lane);
+    __syncthreads();  // every pair row transformed
 
     // split into the two half spectra and store R[tb][k][y][ti] (ti fastest)
     C* __restrict__ R = static_cast<C*>(a.R) + (tb * NXH * static_cast<size_t>(H) + y) * TB;
-    for (int w = tid; w < NXH * TB; w += kThreads) {
+    for (int w = tid; w < NXH * TB; w += NT) {
       const int ti = w & (TB - 1), k = w / TB;
-      const int p = ti >> 1;
-      const C zk = buf[padi(p * NX + k)];
-      const C zn = buf[padi(p * NX + ((NX - k) & (NX - 1)))];
+      const int pp = ti >> 1;
+      const C zk = buf[padi(pp * NX + k)];
+      const C zn = buf[padi(pp * NX + ((NX - k) & (NX - 1)))];
       const C o = (ti & 1) == 0 ? C{(zk.x + zn.x) * half_norm, (zk.y - zn.y) * half_norm}
                                 : C{(zk.y + zn.y) * half_norm, (zn.x - zk.x) * half_norm};
       R[static_cast<uint32_t>(k) * kstride + ti] = o;
@@ -320,10 +320,10 @@ hipError_t rows_np(const RowArgs& a, unsigned grid, hipStream_t s) {
       hipError_t e = hipGetDevice(&dev);
       if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
       if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), kThreads, lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), 64 * NP, lds);
       if (e != hipSuccess) return e;
       const unsigned resident = static_cast<unsigned>(std::max(1, cus) * std::max(1, per_cu));
-      hipLaunchKernelGGL(kernel, dim3(std::min(grid, resident)), dim3(kThreads), lds, s, a, static_cast<uint32_t>(grid));
+      hipLaunchKernelGGL(kernel, dim3(std::min(grid, resident)), dim3(64 * NP), lds, s, a, static_cast<uint32_t>(grid));
       return hipGetLastError();
     }
   }
