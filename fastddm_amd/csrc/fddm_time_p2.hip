@@ -371,8 +371,27 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
         Acc hptot = 0;
 #pragma unroll
         for (int w = 0; w < WPT; ++w) hptot += chunk0[h * WPT + w];
-        auto lag_value = [&](int i) -> Acc {
-          const int d = static_cast<int>(a.lags[i]);
+        // This thread's lag iterations k (index nl - 1 - v - k STEP, at most KL
+        // of them since nl < N/2).  On the plans with registers to spare (the
+        // paired and two-column ones, and FP64, whose LDS allows at most 4
+        // waves per SIMD) their lag values and (cos, sin) entries are loaded
+        // first, so those loads overlap instead of opening every iteration;
+        // the other plans would lose occupancy and load per iteration.
+        constexpr int STEP = Lay::NT / G;
+        constexpr int KL = (NH + STEP - 1) / STEP;
+        constexpr bool PRE = Lay::kPair || SEQ > 1 || F64;
+        int dk[PRE ? KL : 1];
+        C csk[PRE ? KL : 1];
+        if constexpr (PRE) {
+#pragma unroll
+          for (int k = 0; k < KL; ++k) {
+            const int i = a.nl - 1 - v - k * STEP;
+            dk[k] = i >= 0 ? static_cast<int>(a.lags[i]) : 1;
+          }
+#pragma unroll
+          for (int k = 0; k < KL; ++k) csk[k] = tw[TWP + dk[k]];
+        }
+        auto lag_value = [&](int d, C cs) -> Acc {
           Acc D;
           if (SUB && L - d <= kEdge) {
             Acc acc = 0;
@@ -390,7 +409,7 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
             const Acc tl = hptot - htail[d - 1];  // psum(L) - psum(d)
             // N C(d) = Re FFT_N(P)(d) from A = Z(d), B = Z(N/2 - d) and
             // (c, s) = (cos, sin)(2 pi d / N): 2 Re X = A.x + B.x + c (A.y + B.y) - s (A.x - B.x)
-            const C za = hbuf[padi(d)], zb = hbuf[padi((NH - d) & (NH - 1))], cs = tw[TWP + d];
+            const C za = hbuf[padi(d)], zb = hbuf[padi((NH - d) & (NH - 1))];
             const Acc ax = za.x, ay = za.y, bx = zb.x, by = zb.y;
             const Acc re = (ax + bx) + static_cast<Acc>(cs.x) * (ay + by) - static_cast<Acc>(cs.y) * (ax - bx);
             const Acc c = re * (Acc(0.5) * inv_n);
@@ -404,11 +423,12 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
           // is even, so that needs an even row length and an aligned base)
           const size_t o0 = static_cast<size_t>(SEQ) * wgi;
           const bool vec = (nqs & 1) == 0 && reinterpret_cast<uintptr_t>(out) % sizeof(OutPair<OutT>) == 0;
+          static_assert(Lay::KMAX == KL, "one lag iteration count");
 #pragma unroll
-          for (int k = 0; k < Lay::KMAX; ++k) {
-            const int i = a.nl - 1 - v - k * Lay::NT;
+          for (int k = 0; k < KL; ++k) {
+            const int i = a.nl - 1 - v - k * STEP;
             if (i < 0) break;
-            const Acc D = lag_value(i);
+            const Acc D = lag_value(dk[k], csk[k]);
             if (sq == 0) {
               dsave[k] = D;
             } else {
@@ -425,8 +445,19 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
         } else {
           const int ky = qh / a.nxh, kx = qh - ky * a.nxh;
           const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
-          for (int i = a.nl - 1 - v; i >= 0; i -= Lay::NT / G)
-            out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(lag_value(i));
+          if constexpr (PRE) {
+#pragma unroll
+            for (int k = 0; k < KL; ++k) {
+              const int i = a.nl - 1 - v - k * STEP;
+              if (i < 0) break;
+              out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(lag_value(dk[k], csk[k]));
+            }
+          } else {
+            for (int i = a.nl - 1 - v; i >= 0; i -= STEP) {
+              const int d = static_cast<int>(a.lags[i]);
+              out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(lag_value(d, tw[TWP + d]));
+            }
+          }
         }
       }
     }
