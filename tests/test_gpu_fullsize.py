@@ -58,3 +58,48 @@ def test_full_size_properties(hip):
     assert np.allclose(direct[:-2], D[sub - 1], rtol=1e-9, atol=0)
     assert np.allclose(direct[-2:], ref[-2:], rtol=1e-9, atol=0)
     d_img.free()
+
+
+def test_large_size_properties(hip):
+    """4000 frames of 1024 x 1024 uint16 (nt = 8000: the two-column N = 8192
+    time plan, 8-frame blocks whose 64-byte runs take the XCD-aware order,
+    FP64 rows on 4-frame blocks), about 80 log-spaced lags from 1 to 3999:
+    the same properties as above."""
+    Lb, Hb, Wb = 4000, 1024, 1024
+    print(f"large size: {Lb} x {Hb} x {Wb} uint16, log-spaced lags", flush=True)
+    d_img = hip.DeviceBuffer(Lb * Hb * Wb * 2)
+    hip.fill_uniform(d_img, Lb * Hb * Wb, np.uint16, 21, 800, 1200)
+    sub = np.array([1, 37, 1000, Lb - 1])  # also checked by direct differences
+    lags = np.unique(np.concatenate([np.geomspace(1, Lb - 1, 100).astype(int), sub]))
+    win = fwin.blackman_harris((Hb, Wb))
+
+    def run(mode, lg, precision):
+        plan = hip.Plan(mode, np.uint16, (Lb, Hb, Wb), lg, Wb, Hb, 2 * Lb if mode == "fft" else 0, win, precision)
+        d_out = hip.DeviceBuffer(plan.output_size())
+        d_ws = hip.DeviceBuffer(plan.workspace_size())
+        plan.execute(d_img, d_out, d_ws)
+        out = d_out.to_array(plan.out_shape, np.float64)
+        plan.destroy()
+        d_out.free()
+        d_ws.free()
+        return out
+
+    ref = run("fft", lags, "fp64")
+    got = run("fft", lags, "fp32c")
+    assert ref.shape == got.shape == (len(lags) + 2, Hb, Wb // 2 + 1)
+    nz = ref != 0
+    rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
+    assert np.median(rel) <= 1e-5, np.median(rel)
+    assert rel.max() <= 1e-2, rel.max()
+    del got, rel, nz
+
+    D, PS, VAR = ref[:-2], ref[-2], ref[-1]
+    assert np.all(D > 0)
+    assert np.all(VAR >= 0) and np.all(PS >= VAR)
+
+    direct = run("diff", sub, "fp64")
+    idx = np.searchsorted(lags, sub)
+    assert np.all(lags[idx] == sub)
+    assert np.allclose(direct[:-2], D[idx], rtol=1e-9, atol=0)
+    assert np.allclose(direct[-2:], ref[-2:], rtol=1e-9, atol=0)
+    d_img.free()
