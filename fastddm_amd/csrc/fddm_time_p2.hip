@@ -418,9 +418,9 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
           return D;
         };
         if constexpr (SEQ > 1) {
-          // lag iteration k: index nl - 1 - tid - k NT; the second column stores
-          // both values (one access where the pair is aligned: the first column
-          // is even, so that needs an even row length and an aligned base)
+          // the second column stores both values (one access where the pair is
+          // aligned: the first column is even, so that needs an even row
+          // length and an aligned base)
           const size_t o0 = static_cast<size_t>(SEQ) * wgi;
           const bool vec = (nqs & 1) == 0 && reinterpret_cast<uintptr_t>(out) % sizeof(OutPair<OutT>) == 0;
           static_assert(Lay::KMAX == KL, "one lag iteration count");
