@@ -20,6 +20,10 @@ STACKS = [
     (33, 30, 42, "diff", "fp32c"),
     (90, 36, 60, "fft", "fp64"),
     (20, 50, 70, "diff", "fp64"),
+    # long series: the time plans that run two output columns per workgroup
+    # (an even and an odd number of columns)
+    (2500, 12, 10, "fft", "fp32c"),
+    (2300, 9, 8, "fft", "fp64"),
 ]
 
 
