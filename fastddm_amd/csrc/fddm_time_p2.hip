@@ -115,7 +115,6 @@ struct P2Lds {
   static constexpr bool kPair = SUB && TPT >= 256 && TPT <= 512 && 4 * group_bytes <= 160 * 1024;
   static constexpr int NT = TPT < kMinThreads ? kMinThreads : (kPair ? 2 * TPT : TPT);  // workgroup size
   static constexpr int G = NT / TPT;
-  // paired workgroups: two per CU hold 8 waves per SIMD when at most 64 VGPRs are used
   // Compensated plans from N = 8192 hold one q per workgroup (the LDS has no
   // room for a second) and leave registers free: the workgroup transforms
   // the q of two neighbouring output columns one after the other and keeps
@@ -123,8 +122,9 @@ struct P2Lds {
   // lag store writes both columns (16 bytes) at once.
   static constexpr int SEQ = (SUB && !kPair && TPT >= kMinThreads && LOGN >= 13) ? 2 : 1;
   static constexpr int KMAX = (N / 2 + NT - 1) / NT;
-  // registers for the 4 waves per SIMD the LDS allows (FP32; FP64 at N = 8192
-  // has one workgroup of 4 waves per CU)
+  // Register bound: paired workgroups, two per CU, hold 8 waves per SIMD when
+  // at most 64 VGPRs are used; two-column FP32 plans get the 4 waves per SIMD
+  // their LDS allows (FP64 at N = 8192 has one workgroup of 4 waves per CU).
   static constexpr int kMinWaves = kPair ? 8 : (SEQ > 1 && sizeof(T) == 4 ? 4 : 1);
   static constexpr size_t buf = static_cast<size_t>(G) * lds_padded_size(N) * csz;
   static constexpr size_t edge = static_cast<size_t>(G) * 2 * kEdge * csz;
