@@ -99,8 +99,8 @@ struct alignas(2 * sizeof(O)) OutPair {
 
 // Workgroup shape and LDS layout of the time kernel at N = 2^LOGN.  Plans of
 // fewer than kMinThreads threads put G = kMinThreads / TPT q in a workgroup;
-// compensated plans of 256 or 512 threads put two q in one when two such
-// workgroups still fit a CU's LDS.  The q of one workgroup are neighbours, and the lag
+// plans of 256 or 512 threads put two q in one when two such workgroups
+// still fit a CU's LDS.  The q of one workgroup are neighbours, and the lag
 // loop interleaves them across lanes, so each lag store covers G adjacent
 // output values instead of one.
 template <int LOGN, typename T, bool SUB>
@@ -111,8 +111,9 @@ struct P2Lds {
   static constexpr size_t csz = sizeof(cplx_t<T>);
   static constexpr size_t group_bytes =
       lds_padded_size(N) * csz + 2 * kEdge * csz + 2 * WPT * sizeof(double) + WPT * sizeof(Acc);
-  // (the uncompensated FP32 kernel would spill at 8 waves per SIMD: not paired)
-  static constexpr bool kPair = SUB && TPT >= 256 && TPT <= 512 && 4 * group_bytes <= 160 * 1024;
+  // (the uncompensated FP32 N = 4096 kernel spills 12 VGPRs at 8 waves per
+  // SIMD and is still faster paired: 3.28 -> 3.09 ms at 512^2 x 2000)
+  static constexpr bool kPair = TPT >= 256 && TPT <= 512 && 4 * group_bytes <= 160 * 1024;
   static constexpr int NT = TPT < kMinThreads ? kMinThreads : (kPair ? 2 * TPT : TPT);  // workgroup size
   static constexpr int G = NT / TPT;
   // Compensated plans from N = 8192 hold one q per workgroup (the LDS has no

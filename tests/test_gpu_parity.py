@@ -181,11 +181,11 @@ def test_every_time_plan_matches_oracle(hip, L, N, precision):
     CHECK[precision](out, ref)
 
 
-@pytest.mark.parametrize("precision,L", [("fp32c", 700), ("fp32c", 1500), ("fp32c", 2500), ("fp32c", 4500),
-                                         ("fp64", 2500)])
+@pytest.mark.parametrize("precision,L", [("fp32c", 700), ("fp32c", 1500), ("fp32", 1500), ("fp32c", 2500),
+                                         ("fp32c", 4500), ("fp64", 2500)])
 def test_time_plan_odd_number_of_q(hip, precision, L):
     """45 q-series (9 x 8 images): the plans that put several q in one
-    workgroup (two at N = 4096, fp32c) end with a partly empty workgroup
+    workgroup (two at N = 4096, fp32c and fp32) end with a partly empty workgroup
     whose idle groups still serve the lag loop of the others; the plans that
     run two output columns one after the other (N = 8192 and 16384 fp32c,
     N = 8192 fp64) end with a lone column, and with 45 values per lag row
