@@ -29,7 +29,7 @@
 //
 // The output is lag-major, so each lag store of one q writes one value into
 // its own line.  Workgroups of several q interleave them across lanes; the
-// plans from N = 8192, whose LDS holds a single q, instead
+// plans from N = 8192 (FP64 from 4096), whose LDS holds a single q, instead
 // run steps 1-6 for two neighbouring output columns in turn, keep the first
 // column's D in registers and store each lag's two values together.
 #include <type_traits>
@@ -116,12 +116,14 @@ struct P2Lds {
   static constexpr bool kPair = TPT >= 256 && TPT <= 512 && 4 * group_bytes <= 160 * 1024;
   static constexpr int NT = TPT < kMinThreads ? kMinThreads : (kPair ? 2 * TPT : TPT);  // workgroup size
   static constexpr int G = NT / TPT;
-  // Plans from N = 8192 hold one q per workgroup (the LDS has no room for a
-  // second) and leave registers free: the workgroup transforms the q of two
-  // neighbouring output columns one after the other and keeps the first
-  // one's D in registers (KMAX lag iterations per thread), so each lag store
-  // writes both columns at once.  (Plain FP32 at N = 16384 would spill.)
-  static constexpr int SEQ = (!kPair && TPT >= kMinThreads && LOGN >= 13 && (SUB || LOGN == 13)) ? 2 : 1;
+  // Plans from N = 8192 (FP64 from N = 4096) hold one q per workgroup (the
+  // LDS has no room for a second) and leave registers free: the workgroup
+  // transforms the q of two neighbouring output columns one after the other
+  // and keeps the first one's D in registers (KMAX lag iterations per
+  // thread), so each lag store writes both columns at once.  (Plain FP32 at
+  // N = 16384 would spill.)
+  static constexpr int SEQ =
+      (!kPair && TPT >= kMinThreads && (LOGN >= 13 || sizeof(T) == 8) && (SUB || LOGN == 13)) ? 2 : 1;
   static constexpr int KMAX = (N / 2 + NT - 1) / NT;
   // Register bound: paired workgroups, two per CU, hold 8 waves per SIMD when
   // at most 64 VGPRs are used; two-column FP32 plans get the 4 waves per SIMD

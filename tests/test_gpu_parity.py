@@ -182,13 +182,13 @@ def test_every_time_plan_matches_oracle(hip, L, N, precision):
 
 
 @pytest.mark.parametrize("precision,L", [("fp32c", 700), ("fp32c", 1500), ("fp32", 1500), ("fp32c", 2500),
-                                         ("fp32c", 4500), ("fp64", 2500), ("fp32", 2500)])
+                                         ("fp32c", 4500), ("fp64", 1500), ("fp64", 2500), ("fp32", 2500)])
 def test_time_plan_odd_number_of_q(hip, precision, L):
     """45 q-series (9 x 8 images): the plans that put several q in one
     workgroup (two at N = 4096, fp32c and fp32) end with a partly empty workgroup
     whose idle groups still serve the lag loop of the others; the plans that
     run two output columns one after the other (N = 8192 and 16384 fp32c,
-    N = 8192 fp64 and fp32) end with a lone column, and with 45 values per lag row
+    N = 4096 and 8192 fp64, N = 8192 fp32) end with a lone column, and with 45 values per lag row
     the column pairs are stored as two values rather than one aligned pair."""
     img = ar1_speckle(L, 9, 8, sigma=1.1, diffusivity=0.08, mean=700, std=110, seed=L + 7)
     lags = np.arange(1, L)
