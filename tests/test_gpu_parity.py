@@ -242,6 +242,21 @@ def test_no_lags_gives_power_spectrum_and_variance(hip):
         check_fp64(out, ref)
 
 
+@pytest.mark.parametrize("precision,L", [("fp64", 1100), ("fp32c", 2100)])
+def test_no_lags_on_two_column_plans(hip, precision, L):
+    """No lags on the plans that run two output columns per workgroup
+    (FP64 N = 4096, fp32c N = 8192): only PS and VAR are written."""
+    img = ar1_speckle(L, 9, 8, seed=L)
+    ref = onp.ddm_diff(img, [], 8, 9)
+    out = hip.ddm_fft(img, [], 8, 9, next_fast_len(2 * L, "hip"), None, precision=precision)
+    assert out.shape == (2, 9, 5)
+    if precision == "fp64":
+        check_fp64(out, ref)
+    else:  # PS and VAR rows only: the fp32c bar element-wise, and a tight median
+        assert np.isclose(out, ref, rtol=1e-3, atol=0).all()
+        assert np.median(relerr(out, ref)) <= 1e-5
+
+
 def test_two_frames(hip):
     img = ar1_speckle(2, 12, 10, seed=5)
     ref = onp.ddm_diff(img, [1], 10, 12)
