@@ -245,10 +245,19 @@ def _lags_arg(lags) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(lags, dtype=np.uint32))
 
 
-def _window_arg(window: Optional[np.ndarray]):
+def _window_arg(window: Optional[np.ndarray], height: int, width: int):
+    """(pointer, is_float32, keep-alive array) of a window of shape (height, width).
+
+    The library reads height * width values from the pointer, so any other
+    shape is refused here (message of the reference's check, _ddm.py:161-168).
+    """
     if window is None or np.size(window) == 0:
         return None, 0, None
     w = np.ascontiguousarray(window)
+    if w.shape != (int(height), int(width)):
+        raise RuntimeError(
+            f"Window with shape {w.shape} incompatible with image shape {(int(height), int(width))}."
+        )
     if w.dtype == np.float32:
         return w.ctypes.data, 1, w
     w = np.ascontiguousarray(w, dtype=np.float64)
@@ -274,8 +283,8 @@ def ddm_fft(img_seq, lags, nx, ny, nt, window, precision="fp64", out_dtype=np.fl
     """fft mode on the current device; returns (NL+2, ny, nx//2+1) of out_dtype."""
     img = _img_arg(img_seq)
     lg = _lags_arg(lags)
-    wp, wf32, _keep = _window_arg(window)
     L, H, W = img.shape
+    wp, wf32, _keep = _window_arg(window, H, W)
     if not check_host_memory_fft(nx, ny, L, len(lg), out_dtype):
         raise RuntimeError("Not enough space in memory to store the result.\n")
     out = np.empty((len(lg) + 2, ny, nx // 2 + 1), dtype=out_dtype)
@@ -290,8 +299,8 @@ def ddm_diff(img_seq, lags, nx, ny, window, precision="fp64", out_dtype=np.float
     """diff mode on the current device; returns (NL+2, ny, nx//2+1) of out_dtype."""
     img = _img_arg(img_seq)
     lg = _lags_arg(lags)
-    wp, wf32, _keep = _window_arg(window)
     L, H, W = img.shape
+    wp, wf32, _keep = _window_arg(window, H, W)
     if not check_host_memory_diff(nx, ny, L, len(lg), out_dtype):
         raise RuntimeError("Not enough space in memory to store the result.\n")
     out = np.empty((len(lg) + 2, ny, nx // 2 + 1), dtype=out_dtype)
@@ -378,10 +387,11 @@ class Plan:
                  precision="fp64", out_dtype=np.float64):
         self.handle = ctypes.c_void_p()
         lg = _lags_arg(lags)
-        wp, wf32, _keep = _window_arg(window)
         L, H, W = shape
+        wp, wf32, _keep = _window_arg(window, H, W)
         self.mode = mode
         self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
         self.nl = len(lg)
         self.nx, self.ny, self.nt = nx, ny, nt
         self.out_dtype = np.dtype(out_dtype)
@@ -405,8 +415,27 @@ class Plan:
         lib().fddm_plan_output_size(self.handle, ctypes.byref(b))
         return b.value
 
+    def input_size(self) -> int:
+        """Bytes of the (L, H, W) input stack."""
+        L, H, W = self.shape
+        return L * H * W * self.dtype.itemsize
+
+    def _check_buffers(self, d_img, d_ws, d_out=None) -> None:
+        # the kernels index these buffers by the plan's sizes: a short buffer
+        # would be read or written out of bounds on the device
+        for name, buf, need in (("input", d_img, self.input_size()),
+                                ("workspace", d_ws, self.workspace_size()),
+                                ("output", d_out, self.output_size() if d_out is not None else 0)):
+            if buf is None:
+                continue
+            if buf.ptr is None or not buf.ptr.value:
+                raise RuntimeError(f"{name} buffer has been freed")
+            if buf.nbytes < need:
+                raise RuntimeError(f"{name} buffer holds {buf.nbytes} bytes, the plan needs {need}")
+
     def execute(self, d_img: DeviceBuffer, d_out: DeviceBuffer, d_ws: DeviceBuffer,
                 stream: Optional[Stream] = None) -> None:
+        self._check_buffers(d_img, d_ws, d_out)
         _call("fddm_plan_execute", self.handle, d_img.ptr, d_out.ptr, d_ws.ptr,
               stream.handle if stream is not None else None)
 
@@ -422,6 +451,7 @@ class Plan:
     def spectra(self, d_img: DeviceBuffer, d_ws: DeviceBuffer) -> np.ndarray:
         L, H, W = self.shape
         nq = self.ny * (self.nx // 2 + 1)
+        self._check_buffers(d_img, d_ws)
         out = np.empty((nq, L), dtype=np.complex128)
         _call("fddm_plan_spectra", self.handle, d_img.ptr, d_ws.ptr, out.ctypes.data)
         return out
@@ -436,6 +466,11 @@ class Plan:
             self.destroy()
         except Exception:
             pass
+
+
+def synchronize() -> None:
+    """Wait for the work queued on the current device's default stream."""
+    _call("fddm_stream_synchronize", None)
 
 
 def fill_uniform(buf: "DeviceBuffer", count: int, dtype, seed: int, lo: int, hi: int) -> None:
@@ -472,14 +507,17 @@ class Server:
         alive here until release) or, with ``img_on_device``, a DeviceBuffer
         holding a (L, H, W) stack of ``dtype`` given by ``shape``."""
         lg = _lags_arg(lags)
-        wp, wf32, wkeep = _window_arg(window)
         if img_on_device:
             L, H, W = shape
+            need = L * H * W * np.dtype(dtype).itemsize
+            if img.nbytes < need:
+                raise RuntimeError(f"input buffer holds {img.nbytes} bytes, a {shape} stack needs {need}")
             ptr, code = img.ptr, DTYPE_CODES[np.dtype(dtype)]
         else:
             img = _img_arg(img)
             L, H, W = img.shape
             ptr, code = img.ctypes.data, DTYPE_CODES[img.dtype]
+        wp, wf32, wkeep = _window_arg(window, H, W)
         rid = ctypes.c_uint64()
         _call("fddm_server_submit", self._h, MODE_FFT if mode == "fft" else MODE_DIFF, ptr,
               1 if img_on_device else 0, code, L, H, W, lg.ctypes.data, len(lg), nx, ny, nt, wp,
@@ -560,9 +598,15 @@ class Batch:
         self._keep = []
         for k, it in enumerate(items):
             lg = _lags_arg(it["lags"])
-            wp, wf32, wkeep = _window_arg(it.get("window"))
-            self._keep.append((lg, wkeep, it["img"], it["out"]))
             L, H, W = it["shape"]
+            wp, wf32, wkeep = _window_arg(it.get("window"), H, W)
+            self._keep.append((lg, wkeep, it["img"], it["out"]))
+            need_in = L * H * W * np.dtype(it["dtype"]).itemsize
+            need_out = (len(lg) + 2) * it["ny"] * (it["nx"] // 2 + 1) * np.dtype(
+                it.get("out_dtype", np.float64)).itemsize
+            if it["img"].nbytes < need_in or it["out"].nbytes < need_out:
+                raise RuntimeError(f"batch item {k}: input or output buffer smaller than its sizes need "
+                                   f"({it['img'].nbytes} < {need_in} or {it['out'].nbytes} < {need_out})")
             arr[k] = _BatchItem(
                 MODE_FFT if it["mode"] == "fft" else MODE_DIFF, DTYPE_CODES[np.dtype(it["dtype"])],
                 L, H, W, lg.ctypes.data, len(lg), it["nx"], it["ny"], it.get("nt", 0), wp, wf32,

@@ -13,7 +13,8 @@ Behaviour follows ``next_fast_len`` in the reference (src/python/_fftopt.py:29-1
 * any other core raises ``ValueError`` (``:150``).
 
 The search here is written differently (a direct walk over candidates), but
-returns the same integers; the tests check it against brute-force enumeration.
+returns the same integers; tests/test_helpers.py checks it against a brute-force
+enumeration of smooth numbers for every core.
 """
 
 from typing import Optional

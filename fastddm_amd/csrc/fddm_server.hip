@@ -168,6 +168,7 @@ struct fddm_server {
   uint64_t next_id = 1;
   bool stop = false;
   int running = 0;
+  int waiters = 0;  // threads inside fddm_server_wait / fddm_server_next_done
   uint64_t completed = 0;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
@@ -317,12 +318,22 @@ int fddm_server_destroy(fddm_server* s) {
     for (Request* r : s->pending) {
       r->state = kFailed;
       r->error = "server destroyed before admission";
+      r->t_done = s->now();
+      s->finished.push_back(r->id);
     }
     s->pending.clear();
     s->stop = true;
   }
   s->cv_work.notify_all();
+  s->cv_done.notify_all();  // waiters on dropped requests return (failed)
   for (auto& t : s->workers) t.join();
+  {
+    // no thread may still be blocked on (or about to re-lock) the server's
+    // mutex and condition variables when they are freed
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.notify_all();
+    s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+  }
   (void)hipSetDevice(s->device);
   for (auto& kv : s->plans) fddm_plan_destroy(kv.second);
   for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
@@ -413,28 +424,42 @@ int fddm_server_wait(fddm_server* s, uint64_t id, double timeout_s, char* err, s
   if (it == s->requests.end()) return set_err(err, errlen, FDDM_ERR_ARG, "unknown request %llu", (unsigned long long)id);
   Request* r = it->second.get();
   const auto done = [&] { return r->state == kDone || r->state == kFailed; };
+  ++s->waiters;
+  bool ok = true;
   if (timeout_s < 0) {
     s->cv_done.wait(lk, done);
-  } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) {
+  } else {
+    ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done);
+  }
+  const int state = r->state;
+  const std::string error = r->error;
+  if (--s->waiters == 0 && s->stop) s->cv_done.notify_all();
+  if (!ok)
     return set_err(err, errlen, FDDM_ERR_ARG, "request %llu not finished after %.3f s", (unsigned long long)id,
                    timeout_s);
-  }
-  if (r->state == kFailed) return set_err(err, errlen, FDDM_ERR_HIP, "%s", r->error.c_str());
+  if (state == kFailed) return set_err(err, errlen, FDDM_ERR_HIP, "%s", error.c_str());
   return FDDM_OK;
 }
 
 int fddm_server_next_done(fddm_server* s, double timeout_s, uint64_t* id, char* err, size_t errlen) {
   if (!s || !id) return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument to next_done");
   std::unique_lock<std::mutex> lk(s->mu);
-  const auto ready = [&] { return !s->finished.empty(); };
+  const auto ready = [&] { return !s->finished.empty() || s->stop; };
+  ++s->waiters;
+  bool ok = true;
   if (timeout_s < 0) {
     s->cv_done.wait(lk, ready);
-  } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
-    *id = 0;
-    return FDDM_OK;
+  } else {
+    ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready);
   }
-  *id = s->finished.front();
-  s->finished.pop_front();
+  *id = 0;
+  if (ok && !s->finished.empty()) {
+    *id = s->finished.front();
+    s->finished.pop_front();
+  }
+  const bool stopped = s->stop;
+  if (--s->waiters == 0 && s->stop) s->cv_done.notify_all();
+  if (!*id && stopped) return set_err(err, errlen, FDDM_ERR_ARG, "server is shutting down");
   return FDDM_OK;
 }
 
@@ -469,6 +494,8 @@ int fddm_server_release(fddm_server* s, uint64_t id) {
     Request* r = it->second.get();
     if (r->state == kPending || r->state == kRunning) return FDDM_ERR_ARG;
     if (r->out_bytes) s->arena.release(r->out_off, r->out_bytes);
+    // a released id is never reported by fddm_server_next_done
+    s->finished.erase(std::remove(s->finished.begin(), s->finished.end(), id), s->finished.end());
     s->requests.erase(it);
   }
   s->cv_work.notify_all();

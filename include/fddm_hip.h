@@ -158,7 +158,9 @@ int fddm_stream_synchronize(void *stream, char *err, size_t errlen);
 typedef struct fddm_server fddm_server;
 int fddm_server_create(fddm_server **server, int max_slots, uint64_t hbm_bytes, char *err,
                        size_t errlen);
-/* Drops requests not yet admitted, waits for running ones, frees everything. */
+/* Fails the requests not yet admitted (fddm_server_next_done reports them),
+ * waits for the running ones and for every thread blocked in
+ * fddm_server_wait / fddm_server_next_done to return, then frees everything. */
 int fddm_server_destroy(fddm_server *server);
 /* A pending request older than `seconds` blocks later ones from passing it. */
 int fddm_server_set_aging(fddm_server *server, double seconds);
@@ -172,7 +174,8 @@ int fddm_server_status(fddm_server *server, uint64_t id, int *state, double *tim
 /* timeout_s < 0 waits without limit. */
 int fddm_server_wait(fddm_server *server, uint64_t id, double timeout_s, char *err, size_t errlen);
 /* Id of the next request that finished (done or failed), in completion
- * order, each reported once; *id = 0 when timeout_s (>= 0) passes first. */
+ * order, each reported once (released ids are not); *id = 0 when timeout_s
+ * (>= 0) passes first, or with an error once the server is shutting down. */
 int fddm_server_next_done(fddm_server *server, double timeout_s, uint64_t *id, char *err,
                           size_t errlen);
 /* Device pointer and size of a finished result (valid until release). */

@@ -49,8 +49,14 @@ def _shift(out):
 
 def ddm_fft(img, lags, nx, ny, nt, window=None, subtract_mean=False):
     """fft-mode result (NL+2, ny, nx//2+1) float64, y-shifted."""
+    return _shift(structure_from_spectra(spectra(img, nx, ny, window), lags, nt, subtract_mean))
+
+
+def structure_from_spectra(F, lags, nt, subtract_mean=False):
+    """D slices, PS and VAR (NL+2, *F.shape[1:]) of spectra F (L, ...), unshifted.
+
+    The time stage of ``ddm_fft`` on any subset of q (columns of F)."""
     lags = np.asarray(lags, dtype=np.int64)
-    F = spectra(img, nx, ny, window)
     L = F.shape[0]
     ps = np.mean(np.abs(F) ** 2, axis=0)
     var = np.var(F, axis=0)
@@ -65,7 +71,7 @@ def ddm_fft(img, lags, nx, ny, nt, window=None, subtract_mean=False):
         out[i] = (cs[-off] + cs_rev[-off] - 2.0 * ac[lag]) / (L - lag)
     out[-2] = ps
     out[-1] = var
-    return _shift(out)
+    return out
 
 
 def ddm_diff(img, lags, nx, ny, window=None):
