@@ -38,12 +38,36 @@ def log(msg):
 
 
 def algorithmic_bytes(L, H, W, b_in, nq, nl, c, o, window):
-    """Compulsory bytes per stage: inputs read once, outputs written once."""
+    """Compulsory bytes per stage: inputs read once, outputs written once.
+
+    rows: pixels in, row spectra out (L H (W/2+1) complex); columns: row
+    spectra in, q-major spectra out; time: q-major spectra in, D/PS/VAR out."""
+    nxh = W // 2 + 1
     spectra = L * nq * c
     out = (nl + 2) * nq * o
-    spatial = L * H * W * b_in + spectra + (H * W * (c // 2) if window else 0)
+    rows = L * H * W * b_in + L * H * nxh * c + (H * W * (c // 2) if window else 0)
+    cols = L * H * nxh * c + spectra
     time_stage = spectra + out
-    return {"spatial": spatial, "time": time_stage, "whole": spatial + time_stage}
+    whole = L * H * W * b_in + 2 * spectra + out
+    return {"rows": rows, "cols": cols, "spatial": L * H * W * b_in + spectra, "time": time_stage, "whole": whole}
+
+
+def stage_fractions(st, b):
+    """Algorithmic bytes / stage time / peak for the row, column and time stages."""
+    fr = {}
+    for key, ms_key in (("rows", "row_pass"), ("cols", "column_pass"), ("time", "time_or_diff")):
+        ms = st.get(ms_key)
+        if ms:
+            gbs = b[key] / (ms / 1e3) / 1e9
+            fr[key] = {"GB": b[key] / 1e9, "ms": ms, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return fr
+
+
+def _time_transform_length(L):
+    n = 512
+    while n < 2 * L - 1:
+        n *= 2
+    return n
 
 
 def run_gpu(core, img, lags, nx, ny, nt, win, precision, steps, warmup, keep_output, mode="fft"):
@@ -86,39 +110,97 @@ def run_gpu(core, img, lags, nx, ny, nt, win, precision, steps, warmup, keep_out
     return wall / steps * 1e3, stage_avg, out
 
 
-def cpu_baseline(img, lags, nx, ny, nt, win, full_L, nq):
-    """Single-thread C restatement on a sample, scaled to the full job."""
+def _cpu_model():
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(img, lags, nx, ny, nt, win, full_L, nq, threads):
+    """FP64 C restatement of the reference cpp core on a sample of the
+    benchmark workload, scaled to the full job (frames/s)."""
     from oracle import port
 
-    if not os.path.isfile(port._LIB_PATH):
-        port.build()
     L, H, W = img.shape
     # spatial stage: 100 of the frames at full size
     fs = min(100, L)
     t0 = time.perf_counter()
-    port.rfft2(img[:fs], nx, ny, win, threads=1)
+    port.rfft2(img[:fs], nx, ny, win, threads=threads)
     t_spatial = (time.perf_counter() - t0) / fs * full_L
     # time stage: all frames, a 16-row crop (16 * (nx/2+1) q-series)
     crop = np.ascontiguousarray(img[:, :16, :])
     wcrop = None if win is None else np.ascontiguousarray(win[:16, :])
     nq_crop = 16 * (nx // 2 + 1)
     t0 = time.perf_counter()
-    port.rfft2(crop, nx, 16, wcrop, threads=1)
+    port.rfft2(crop, nx, 16, wcrop, threads=threads)
     t_crop_spatial = time.perf_counter() - t0
     t0 = time.perf_counter()
-    port.ddm_fft(crop, lags, nx, 16, nt, wcrop, chunk_size=257, threads=1)
+    port.ddm_fft(crop, lags, nx, 16, nt, wcrop, chunk_size=257, threads=threads)
     t_crop_total = time.perf_counter() - t0
     t_time = max(t_crop_total - t_crop_spatial, 0.0) / nq_crop * nq
     total = t_spatial + t_time
     return {
         "value": full_L / total,
         "unit": "frames/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"spatial stage timed on {fs} of {full_L} frames (512x512), time stage on all "
+        "sample": (f"spatial stage timed on {fs} of {full_L} frames ({H}x{W}), time stage on all "
                    f"{full_L} frames for {nq_crop} of {nq} q-series (16-row crop); both scaled "
                    f"linearly to the full job; est. {total:.1f} s per analysis"),
     }
+
+
+def cpu_full_configs(threads):
+    """The plain-method (64 x 128^2, fft, lags 1..63) and diff-mode (300 x 256^2
+    uint8, 50 log-spaced lags) workloads run in full by the C restatement."""
+    from fastddm_amd.lags import logspace_int
+    from fastddm_amd.speckle import CONFIGS, ar1_speckle
+    from oracle import port
+
+    res = {}
+    img = ar1_speckle(**CONFIGS["cpu_128"])
+    L = img.shape[0]
+    t0 = time.perf_counter()
+    port.ddm_fft(img, np.arange(1, L), 128, 128, 128, None, chunk_size=65 * 128, threads=threads)
+    dt = time.perf_counter() - t0
+    res["fft_64x128x128"] = {"s": dt, "frames_per_s": L / dt}
+    img = ar1_speckle(**CONFIGS["diff_256"])
+    L = img.shape[0]
+    t0 = time.perf_counter()
+    port.ddm_diff(img, logspace_int(L - 1, num=50, endpoint=True), 256, 256, None, threads=threads)
+    dt = time.perf_counter() - t0
+    res["diff_300x256x256"] = {"s": dt, "frames_per_s": L / dt}
+    return res
+
+
+def cpu_baselines(img, lags, nx, ny, nt, win, L, nq):
+    """cpu_baseline of the bench line: the 1-thread restatement (the reference
+    cpp core is single-threaded) plus the all-core run and the two small
+    workloads in full, with the host CPU named."""
+    from oracle import port
+
+    if not os.path.isfile(port._LIB_PATH):
+        port.build()
+    allc = max(1, min(16, len(os.sched_getaffinity(0))))
+    base = cpu_baseline(img, lags, nx, ny, nt, win, L, nq, threads=1)
+    base["cpu_model"] = _cpu_model()
+    base["all_cores"] = cpu_baseline(img, lags, nx, ny, nt, win, L, nq, threads=allc)
+    base["full_workloads"] = {"threads_1": cpu_full_configs(1), f"threads_{allc}": cpu_full_configs(allc)}
+    return base
 
 
 def batch_main(args):
@@ -355,6 +437,123 @@ def serve_main(args):
     }
 
 
+def _subset_rows(nl, L, lags):
+    """Output rows compared in the large leg: every 16th lag, every lag with
+    at most 64 frame pairs, PS and VAR."""
+    lags = np.asarray(lags)
+    rows = set(range(0, nl, 16)) | set(np.nonzero(L - lags <= 64)[0].tolist()) | {nl, nl + 1}
+    return np.array(sorted(rows))
+
+
+def large_leg(core, args):
+    """The north-star size: 4000 frames of 1024x1024 uint16 AR(1) speckle
+    (sigma 1.5, D 0.2, mean 1000, std 150, seed 6), lags 1..3999,
+    Blackman-Harris, nt 8000, float64 output, input resident in HBM; fp32c
+    and fp64 timed with per-stage HIP events, fp32c error against fp64 on a
+    subset of output rows."""
+    from fastddm_amd.speckle import ar1_speckle
+    from fastddm_amd.window import blackman_harris
+
+    L, S = 4000, 1024
+    t0 = time.perf_counter()
+    img = ar1_speckle(L, S, S, sigma=1.5, diffusivity=0.2, mean=1000.0, std=150.0, dtype=np.uint16, seed=6)
+    log(f"large leg: generated {img.shape} speckle in {time.perf_counter() - t0:.1f} s")
+    win = blackman_harris((S, S))
+    lags = np.arange(1, L)
+    nt = 2 * L
+    nxh = S // 2 + 1
+    nq = S * nxh
+    d_img = core.DeviceBuffer.from_array(img)
+    del img
+    rows = _subset_rows(len(lags), L, lags)
+    res = {"workload": (f"fft mode, {L} frames of {S}x{S} uint16 speckle, lags 1..{L - 1}, Blackman-Harris, "
+                        f"nt={nt} (time transform length {_time_transform_length(L)}), float64 output"),
+           "frames": L, "height": S, "width": S}
+    outs = {}
+    for prec, steps in (("fp32c", args.large_steps), ("fp64", max(2, args.large_steps // 2))):
+        plan = core.Plan("fft", np.uint16, (L, S, S), lags, S, S, nt, win, prec)
+        d_out = core.DeviceBuffer(plan.output_size())
+        d_ws = core.DeviceBuffer(plan.workspace_size())
+        stream = core.Stream()
+        plan.execute(d_img, d_out, d_ws, stream)
+        stream.synchronize()
+        plan.timing_begin(steps)
+        t = time.perf_counter()
+        for _ in range(steps):
+            plan.execute(d_img, d_out, d_ws, stream)
+        stream.synchronize()
+        ms = (time.perf_counter() - t) / steps * 1e3
+        st, n = plan.timing_end()
+        st = {k: v / max(n, 1) for k, v in st.items()}
+        row_bytes = nq * 8
+        outs[prec] = np.stack([d_out.read(int(r) * row_bytes, (nq,), np.float64) for r in rows])
+        c = 8 if prec == "fp32c" else 16
+        b = algorithmic_bytes(L, S, S, 2, nq, len(lags), c, 8, True)
+        res[prec] = {"ms_per_step": ms, "frames_per_s": L / (ms / 1e3), "stages_ms": st,
+                     "stage_hbm": stage_fractions(st, b),
+                     "whole_path_frac": b["whole"] / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        log(f"large leg {prec}: {ms:.2f} ms/step, stages {st}")
+        for buf in (d_out, d_ws):
+            buf.free()
+        stream.destroy()
+        plan.destroy()
+    d_img.free()
+    a, r = outs["fp32c"], outs["fp64"]
+    m = r != 0
+    rel = np.abs(a[m] - r[m]) / np.abs(r[m])
+    res["speedup_fp32c_vs_fp64"] = res["fp64"]["ms_per_step"] / res["fp32c"]["ms_per_step"]
+    res["fp32c_vs_fp64_rel_err"] = {"median": float(np.median(rel)), "max": float(rel.max()),
+                                   "rows_compared": int(len(rows)),
+                                   "rows": "every 16th lag, every lag with <= 64 frame pairs, PS, VAR"}
+    return res
+
+
+def diff_leg(core, args):
+    """Diff mode: 300 frames of 256x256 uint8 speckle (sigma 2, D 1, mean 120,
+    std 25, clipped 0..255, seed 3), 50 log-spaced lags in 1..299, no window,
+    fp32c and fp64, input resident in HBM."""
+    from fastddm_amd.lags import logspace_int
+    from fastddm_amd.speckle import CONFIGS, ar1_speckle
+
+    img = ar1_speckle(**CONFIGS["diff_256"])
+    L, H, W = img.shape
+    lags = logspace_int(L - 1, num=50, endpoint=True)
+    nq = H * (W // 2 + 1)
+    res = {"workload": f"diff mode, {L} frames of {H}x{W} uint8 speckle, 50 log-spaced lags in 1..{L - 1}, no window"}
+    outs = {}
+    for prec in ("fp32c", "fp64"):
+        ms, st, out = run_gpu(core, img, lags, W, H, 0, None, prec, args.steps, args.warmup, keep_output=True,
+                              mode="diff")
+        outs[prec] = out
+        pairs = nq * int(np.sum(L - lags))
+        tflops = 6.0 * pairs / (st["time_or_diff"] / 1e3) / 1e12
+        c = 8 if prec == "fp32c" else 16
+        b = algorithmic_bytes(L, H, W, 1, nq, len(lags), c, 8, False)
+        res[prec] = {"ms_per_step": ms, "frames_per_s": L / (ms / 1e3), "stages_ms": st,
+                     "stage_hbm": stage_fractions(st, b),
+                     "valu": {"kernel": "diff", "achieved": tflops, "peak": FP64_VECTOR_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": tflops / FP64_VECTOR_PEAK_TFLOPS, "differences": pairs}}
+        log(f"diff leg {prec}: {ms:.3f} ms/step, stages {st}")
+    m = outs["fp64"] != 0
+    rel = np.abs(outs["fp32c"][m] - outs["fp64"][m]) / np.abs(outs["fp64"][m])
+    res["fp32c_vs_fp64_rel_err"] = {"median": float(np.median(rel)), "max": float(rel.max())}
+    return res
+
+
+def _traffic_entry(dom, L, S):
+    """HBM traffic per launch of the dominant kernel, from the stored rocprofv3
+    counter run (not measured in this run; the entry names its source)."""
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if not (os.path.isfile(tf) and dom == "time_wk" and (L, S) == (2000, 512)):
+        return None, None
+    with open(tf) as f:
+        t = json.load(f)
+    entry = t.get("k_time_fp32c_512x2000")
+    if not entry:
+        return None, None
+    return entry.get("bytes_per_launch"), entry.get("source")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -368,6 +567,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--skip-large", action="store_true", help="skip the 1024^2 x 4000 leg")
+    ap.add_argument("--skip-diff-leg", action="store_true", help="skip the diff-mode leg")
+    ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving measurement")
     ap.add_argument("--precision-report", action="store_true",
                     help="only report fp64 / fp32 / fp32c errors against the FP64 oracle")
@@ -432,7 +634,7 @@ def main():
     t0 = time.perf_counter()
     if diff:
         img = ar1_speckle(L, S, S, sigma=2.0, diffusivity=1.0, mean=120.0, std=25.0,
-                          dtype=np.uint8, seed=3)
+                          dtype=np.uint8, seed=3, clip=(0, 255))
         win = None
         lags = logspace_int(L - 1, num=50, endpoint=True)
     else:
@@ -462,8 +664,10 @@ def main():
             core.ddm_fft(img, lags, nx, ny, nt, win, precision="fp32c")
         e2e.append(time.perf_counter() - t0)
     e2e_s = min(e2e[1:])
+    core.release_cache()
     log(f"host-to-host fp32c: {e2e_s * 1e3:.1f} ms per call")
     fp64 = None
+    b64 = algorithmic_bytes(L, S, S, img.itemsize, nq, len(lags), 16, 8, win is not None)
     if not args.skip_fp64:
         ms64, st64, out64 = run_gpu(core, img, lags, nx, ny, nt, win, "fp64", args.steps,
                                     args.warmup, keep_output=True, mode=mode)
@@ -475,6 +679,7 @@ def main():
             "frames_per_s": L / (ms64 / 1e3),
             "ms_per_step": ms64,
             "stages_ms": st64,
+            "stage_hbm": stage_fractions(st64, b64),
             "fp32c_vs_fp64_gpu_rel_err": {"median": float(np.median(rel)), "max": float(rel.max())},
         }
         del out32, out64, d32, d64, rel
@@ -482,19 +687,14 @@ def main():
     b = algorithmic_bytes(L, S, S, img.itemsize, nq, len(lags), 8, 8, win is not None)
     kernels = {
         ("diff" if diff else "time_wk"): (st32["time_or_diff"], b["time"]),
-        "spatial_rows_cols": (st32["row_pass"] + st32["column_pass"], b["spatial"]),
+        "rows": (st32["row_pass"], b["rows"]),
+        "columns": (st32["column_pass"], b["cols"]),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
     dom_ms, dom_bytes = kernels[dom]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.isfile(tf):
-        with open(tf) as f:
-            t = json.load(f)
-        entry = t.get("k_time_fp32c_512x2000") if (dom == "time_wk" and (L, S) == (2000, 512)) else None
-        if entry:
-            traffic = entry.get("bytes_per_launch")
+    traffic, traffic_src = _traffic_entry(dom, L, S)
+    tlen = None if diff else _time_transform_length(L)
 
     result = {
         "metric": "fft-mode frames/s + HBM roofline %, 512^2x2000 fp32 vs fp64; serve p50/p99",
@@ -514,11 +714,13 @@ def main():
             "workload": (f"diff mode, {L} frames of {S}x{S} uint8, {len(lags)} log-spaced lags in "
                          f"1..{L - 1}, no window, precision fp32c, float64 output" if diff else
                          f"fft mode, {L} frames of {S}x{S} uint16, lags 1..{L - 1}, Blackman-Harris "
-                         f"window, nt={nt}, precision fp32c, float64 output"),
-            "frames": L, "height": S, "width": S, "nt": nt, "lags": len(lags),
-            "precision": "fp32c", "parallelism": "single GPU",
+                         f"window, nt={nt} (time transform run at length {tlen}: D is the same for "
+                         f"any zero-padded length >= 2L-1), precision fp32c, float64 output"),
+            "frames": L, "height": S, "width": S, "nt": nt, "time_transform_length": tlen,
+            "lags": len(lags), "precision": "fp32c", "parallelism": "single GPU",
         },
         "stages_ms": st32,
+        "stage_hbm": stage_fractions(st32, b),
         "roofline": {
             "kernel": dom,
             "bound": "hbm",
@@ -527,6 +729,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes": dom_bytes,
         },
         "host_to_host": {
@@ -554,10 +757,16 @@ def main():
     if fp64 is not None:
         result["fp64"] = fp64
         result["speedup_vs_fp64"] = fp64["ms_per_step"] / ms32
+    if not diff and not args.skip_large and args.frames is None and args.size is None:
+        t0 = time.perf_counter()
+        result["large_1024x4000"] = large_leg(core, args)
+        log(f"large leg: {time.perf_counter() - t0:.1f} s")
+    if not diff and not args.skip_diff_leg and args.frames is None and args.size is None:
+        result["diff_300x256x256"] = diff_leg(core, args)
     if not args.skip_cpu and not diff:
         t0 = time.perf_counter()
-        result["cpu_baseline"] = cpu_baseline(img, lags, nx, ny, nt, win, L, nq)
-        log(f"cpu baseline sample took {time.perf_counter() - t0:.1f} s")
+        result["cpu_baseline"] = cpu_baselines(img, lags, nx, ny, nt, win, L, nq)
+        log(f"cpu baselines took {time.perf_counter() - t0:.1f} s")
     print(json.dumps(result), flush=True)
 
 

@@ -58,6 +58,8 @@ def lib() -> ctypes.CDLL:
         "fddm_ddm_fft": [p, i32, u64, u64, u64, p, u64, u64, u64, u64, p, i32, i32, i32, p, cp, sz],
         "fddm_ddm_diff": [p, i32, u64, u64, u64, p, u64, u64, u64, p, i32, i32, i32, p, cp, sz],
         "fddm_release_cache": [cp, sz],
+        "fddm_set_memory_budget": [u64, cp, sz],
+        "fddm_get_last_call_chunks": [ctypes.POINTER(u64), cp, sz],
         "fddm_plan_create": [ctypes.POINTER(p), i32, i32, u64, u64, u64, p, u64, u64, u64, u64,
                              p, i32, i32, i32, cp, sz],
         "fddm_plan_destroy": [p],
@@ -115,6 +117,8 @@ EXPORTED_SYMBOLS = (
     "fddm_ddm_fft",
     "fddm_ddm_diff",
     "fddm_release_cache",
+    "fddm_set_memory_budget",
+    "fddm_get_last_call_chunks",
     "fddm_plan_create",
     "fddm_plan_destroy",
     "fddm_plan_workspace_size",
@@ -231,6 +235,22 @@ def release_cache() -> None:
     _call("fddm_release_cache")
 
 
+def set_memory_budget(nbytes: int) -> None:
+    """Cap the device memory of ``ddm_fft`` / ``ddm_diff`` on the current
+    device (0 restores the default, 95 % of free memory).  Larger analyses
+    then run in chunks of frames and of q, with the same result."""
+    _call("fddm_set_memory_budget", int(nbytes))
+
+
+def last_call_chunks() -> dict:
+    """How the last ``ddm_fft`` / ``ddm_diff`` call on the current device ran:
+    mode 0 (one launch per stage), 1 (pieces, spectra on the device) or 2
+    (pieces, spectra in host memory), and its frame and q chunk counts."""
+    v = (ctypes.c_uint64 * 3)()
+    _call("fddm_get_last_call_chunks", v)
+    return {"mode": int(v[0]), "frame_chunks": int(v[1]), "q_chunks": int(v[2])}
+
+
 # ---- argument preparation ----
 def _img_arg(img_seq: np.ndarray) -> np.ndarray:
     img = np.ascontiguousarray(img_seq)
@@ -342,6 +362,14 @@ class DeviceBuffer:
         _call("fddm_memcpy_dtoh", out.ctypes.data, self.ptr, out.nbytes)
         return out
 
+    def read(self, offset: int, shape, dtype) -> np.ndarray:
+        """Copy ``prod(shape)`` elements of ``dtype`` starting ``offset`` bytes in."""
+        out = np.empty(shape, dtype=dtype)
+        if offset < 0 or offset + out.nbytes > self.nbytes:
+            raise RuntimeError("read outside the device buffer")
+        _call("fddm_memcpy_dtoh", out.ctypes.data, ctypes.c_void_p(self.ptr.value + int(offset)), out.nbytes)
+        return out
+
     def copy_from(self, other: "DeviceBuffer", nbytes: int) -> None:
         """Device-to-device copy of nbytes from `other` into this buffer."""
         _call("fddm_memcpy_dtod", self.ptr, other.ptr, int(nbytes))
@@ -381,7 +409,7 @@ class Stream:
 class Plan:
     """A reusable execution plan for device-resident stacks of one shape."""
 
-    STAGES = ("frame0_spectra", "row_pass", "column_pass", "time_or_diff")
+    STAGES = ("frame0_spectra", "row_pass", "column_pass", "time_or_diff", "edge_fp64")
 
     def __init__(self, mode, dtype, shape, lags, nx, ny, nt=0, window=None,
                  precision="fp64", out_dtype=np.float64):
@@ -443,7 +471,7 @@ class Plan:
         _call("fddm_plan_timing_begin", self.handle, int(max_steps))
 
     def timing_end(self):
-        ms = (ctypes.c_double * 4)()
+        ms = (ctypes.c_double * len(self.STAGES))()
         steps = ctypes.c_int(0)
         _call("fddm_plan_timing_end", self.handle, ms, ctypes.byref(steps))
         return dict(zip(self.STAGES, list(ms))), steps.value

@@ -31,6 +31,9 @@ struct fddm_plan {
   uint64_t L = 0, H = 0, W = 0, nx = 0, ny = 0, nt = 0, nxh = 0, nq = 0, nl = 0;
   int Tb = 2, Gx = 1, Gy = 1;
   int logn_t = 0;  // time transform length 2^logn_t (fft mode)
+  bool long_t = false;  // four-step time transform with device scratch (fddm_time_long.hip)
+  int long_slots = 0;
+  uint64_t long_slot = 0, bytes_long = 0;
   int logn_x = 0;  // > 0: power-of-two row kernel
   int logn_y = 0;  // > 0: power-of-two column kernel
   uint64_t Lp = 0, nblocks = 0;
@@ -41,7 +44,13 @@ struct fddm_plan {
   void* d_twx = nullptr;
   void* d_twy = nullptr;
   void* d_twt = nullptr;
-  uint64_t bytes_R = 0, bytes_S = 0, bytes_S0 = 0, ws_bytes = 0;
+  uint64_t bytes_R = 0, bytes_S = 0, bytes_S0 = 0, bytes_flags = 0, ws_bytes = 0;
+  // Compensated FP32: child FP64 plan over the first and last kEdgeFrames
+  // frames (gathered into its own input unless the stack is that short); the
+  // lags from index edge_i0 on have at most kEdgeFrames pairs.
+  fddm_plan* edge = nullptr;
+  uint64_t edge_frames = 0, bytes_edge_in = 0, bytes_edge_ws = 0;
+  int edge_i0 = 0;
   // timing
   std::vector<hipEvent_t> events;
   int timing_cap = 0;
@@ -50,7 +59,8 @@ struct fddm_plan {
 
 namespace {
 
-constexpr int kEventsPerStep = 5;
+constexpr int kStages = 5;
+constexpr int kEventsPerStep = kStages + 1;
 constexpr size_t kLdsLimit = 160 * 1024 - 1024;
 
 int set_err(char* err, size_t len, int code, const char* fmt, ...) {
@@ -222,6 +232,11 @@ int upload_twiddles(void** dst, const FftDesc& d, bool f64, char* err, size_t er
 }
 
 void plan_free_device(fddm_plan* p) {
+  if (p->edge) {
+    plan_free_device(p->edge);
+    delete p->edge;
+    p->edge = nullptr;
+  }
   for (void* ptr : {p->d_lags, p->d_window, p->d_twx, p->d_twy, p->d_twt})
     if (ptr) (void)hipFree(ptr);
   p->d_lags = p->d_window = p->d_twx = p->d_twy = p->d_twt = nullptr;
@@ -323,10 +338,14 @@ int fddm::plan_create_ex(fddm_plan** out, int mode, int dtype, uint64_t L, uint6
     int logn = kP2MinLog;
     while ((1ull << logn) < 2 * L - 1) ++logn;
     const int maxlog = f64 ? kP2MaxLog - 1 : kP2MaxLog;
-    if (logn > maxlog || p2_lds_bytes(logn, precision) > kLdsLimit)
-      return fail(set_err(err, errlen, FDDM_ERR_SIZE,
-                          "nt=%llu too long for the LDS-resident time kernel (at most %d frames in %s)",
-                          (unsigned long long)nt, 1 << (maxlog - 1), f64 ? "fp64" : "fp32"));
+    if (logn > maxlog || p2_lds_bytes(logn, precision) > kLdsLimit) {
+      // longer series: four-step transform through device scratch
+      const int lmax = long_max_log(precision);
+      if (logn > lmax)
+        return fail(set_err(err, errlen, FDDM_ERR_SIZE, "nt=%llu too long for the time kernels (at most %d frames in %s)",
+                            (unsigned long long)nt, 1 << (lmax - 1), f64 ? "fp64" : "fp32"));
+      p->long_t = true;
+    }
     p->logn_t = logn;
   }
 
@@ -353,7 +372,16 @@ int fddm::plan_create_ex(fddm_plan** out, int mode, int dtype, uint64_t L, uint6
     return fail(rc);
   if ((rc = p->logn_y ? upload_sp(&p->d_twy, p->logn_y, true) : upload_twiddles(&p->d_twy, p->dy, f64, err, errlen)))
     return fail(rc);
-  if (mode == MODE_FFT) {
+  if (mode == MODE_FFT && p->long_t) {
+    const std::vector<double> wd = long_twiddles(p->logn_t);
+    if (f64) {
+      rc = upload(&p->d_twt, wd.data(), wd.size() * sizeof(double), err, errlen);
+    } else {
+      std::vector<float> wf(wd.begin(), wd.end());
+      rc = upload(&p->d_twt, wf.data(), wf.size() * sizeof(float), err, errlen);
+    }
+    if (rc) return fail(rc);
+  } else if (mode == MODE_FFT) {
     if (f64) {
       auto w = p2_twiddles<double>(p->logn_t);
       rc = upload(&p->d_twt, w.data(), w.size() * sizeof(double), err, errlen);
@@ -383,7 +411,28 @@ int fddm::plan_create_ex(fddm_plan** out, int mode, int dtype, uint64_t L, uint6
   p->bytes_R = align256(p->nblocks * p->nxh * H * p->Tb * csz);
   p->bytes_S = align256(p->nq * p->Lp * csz);
   p->bytes_S0 = p->subtract0 ? align256(p->nq * p->Tb * csz) : 0;
-  p->ws_bytes = p->bytes_R + p->bytes_S + p->bytes_S0;
+  // fft mode, compensated: one byte per q marks lags for the direct recomputation
+  p->bytes_flags = (mode == MODE_FFT && precision == PREC_FP32C) ? align256(p->nq) : 0;
+  if (p->long_t) {
+    // one scratch slot per resident workgroup of the four-step kernel
+    p->long_slots = static_cast<int>(std::min<uint64_t>(p->nq, f64 ? 512 : 1024));
+    p->long_slot = long_slot_bytes(p->logn_t, precision, L);
+    p->bytes_long = align256(p->long_slot * static_cast<uint64_t>(p->long_slots));
+  }
+  p->ws_bytes = p->bytes_R + p->bytes_S + p->bytes_S0 + p->bytes_flags + p->bytes_long;
+  if (precision == PREC_FP32C && nl > 0 && L - lags[nl - 1] <= static_cast<uint64_t>(kEdgeFrames)) {
+    // FP64 spectra of the edge frames for the lags with few pairs
+    const uint64_t E = kEdgeFrames;
+    p->edge_frames = std::min<uint64_t>(L, 2 * E);
+    if ((rc = plan_create_ex(&p->edge, MODE_DIFF, dtype, p->edge_frames, H, W, nullptr, 0, nx, ny, 0, window,
+                             window_f32, PREC_FP64, OUT_F64, 0, err, errlen)))
+      return fail(rc);
+    p->bytes_edge_in = L > 2 * E ? align256(2 * E * H * W * in_bytes(dtype)) : 0;
+    p->bytes_edge_ws = p->edge->ws_bytes;
+    p->ws_bytes += p->bytes_edge_in + p->bytes_edge_ws;
+    p->edge_i0 = static_cast<int>(nl);
+    while (p->edge_i0 > 0 && L - lags[p->edge_i0 - 1] <= E) --p->edge_i0;
+  }
   *out = p;
   return FDDM_OK;
 }
@@ -408,6 +457,7 @@ int run_spatial(fddm_plan* p, const void* d_img, void* ws, hipStream_t s, char* 
   const double norm = 1.0 / std::sqrt(static_cast<double>(p->nx) * static_cast<double>(p->ny));
   RowArgs ra{};
   ra.img = d_img;
+  ra.img0 = d_img;
   ra.in_dtype = p->in_dtype;
   ra.L = p->L;
   ra.H = static_cast<int>(p->H);
@@ -475,12 +525,38 @@ int plan_execute(fddm_plan* p, const void* d_img, void* d_out, void* ws, hipStre
   ta.dt = p->dt;
   ta.twt = p->d_twt;
   ta.out = d_out;
-  if (p->mode == MODE_FFT) {
+  ta.flags = p->bytes_flags ? base + p->bytes_R + p->bytes_S + p->bytes_S0 : nullptr;
+  time_full(ta);
+  if (p->mode == MODE_FFT && p->long_t) {
+    FDDM_HIP(launch_time_long(ta, p->logn_t, p->precision, p->out_dtype,
+                              base + p->bytes_R + p->bytes_S + p->bytes_S0 + p->bytes_flags, p->long_slot,
+                              p->long_slots, s));
+  } else if (p->mode == MODE_FFT) {
     FDDM_HIP(launch_time_p2(ta, p->logn_t, p->precision, p->out_dtype, s));
   } else {
     FDDM_HIP(launch_diff(ta, p->precision, p->out_dtype, s));
   }
   if ((rc = record(p, 4, s, err, errlen))) return rc;
+  // compensated FP32 corrections: flagged Wiener-Khinchin lags, then the lags
+  // with at most kEdgeFrames pairs from FP64 spectra
+  if (ta.flags) FDDM_HIP(launch_wk_fix(ta, p->out_dtype, s));
+  if (p->edge) {
+    // FP64 spectra of the edge frames, then the few-pair lags from them
+    unsigned char* ein = base + p->ws_bytes - p->bytes_edge_ws - p->bytes_edge_in;
+    unsigned char* ews = base + p->ws_bytes - p->bytes_edge_ws;
+    const void* src = d_img;
+    if (p->bytes_edge_in) {
+      FDDM_HIP(launch_gather_edges(d_img, ein, p->H * p->W * in_bytes(p->in_dtype), p->L, kEdgeFrames, s));
+      src = ein;
+    }
+    const int cap = p->edge->timing_cap;
+    p->edge->timing_cap = 0;
+    rc = run_spatial(p->edge, src, ews, s, err, errlen);
+    p->edge->timing_cap = cap;
+    if (rc) return rc;
+    FDDM_HIP(launch_edge_fix(ta, ews + p->edge->bytes_R, p->edge->Lp, kEdgeFrames, p->edge_i0, p->out_dtype, s));
+  }
+  if ((rc = record(p, 5, s, err, errlen))) return rc;
   if (p->timing_cap > 0 && p->timing_steps < p->timing_cap) ++p->timing_steps;
   return FDDM_OK;
 }
@@ -559,7 +635,7 @@ int d2h_staged(HostPathCache& c, void* h_dst, const void* d_src, size_t n, char*
   }
   return FDDM_OK;
 }
-std::mutex g_cache_mu;
+std::mutex g_cache_mu[64];  // one per device: calls on different devices do not wait for each other
 HostPathCache g_cache[64];
 
 int grow(void** ptr, uint64_t* cap, uint64_t need, char* err, size_t errlen) {
@@ -569,6 +645,251 @@ int grow(void** ptr, uint64_t* cap, uint64_t need, char* err, size_t errlen) {
   *cap = 0;
   FDDM_HIP(hipMalloc(ptr, need ? need : 1));
   *cap = need;
+  return FDDM_OK;
+}
+
+// ---- host path within a device-memory budget ----
+//
+// When the one-shot footprint (input + workspace + output) exceeds the
+// budget, the call runs in pieces, as the reference's planner does with its
+// loop counts (memchk_gpu.cu:141-702, ddm_cuda.cu:40-261, :535-923):
+//   * the spatial stage over chunks of whole frame blocks: each chunk's frames
+//     are copied in and transformed (frame 0, the subtracted frame, stays on
+//     the device), and its columns land in the q-major spectra at their
+//     frame offset -- in device memory (mode 1) or, when the spectra
+//     themselves do not fit, in host memory through a device piece (mode 2);
+//   * the time stage over chunks of whole ky rows (q ranges), each stored
+//     unshifted in a device piece and copied to its y-shifted rows of the
+//     host result (two contiguous runs at most).
+// Every kernel computes each (frame block, row) and each q exactly as in one
+// launch, so the result is bit-identical to the one-shot path.
+uint64_t g_budget[64] = {};
+uint64_t g_last_stats[64][3] = {};
+
+struct ChunkPlan {
+  int mode = 0;                 // 1: spectra on the device, 2: spectra on the host
+  uint64_t frames_per = 0;      // frames per spatial chunk (multiple of Tb)
+  uint64_t rows_per = 0;        // ky rows per time chunk
+  uint64_t fixed = 0, work = 0;  // device bytes
+};
+
+bool plan_chunks(const fddm_plan* p, uint64_t budget, ChunkPlan* cp) {
+  const uint64_t csz = p->precision == PREC_FP64 ? 16 : 8;
+  const uint64_t osz = p->out_dtype == OUT_F32 ? 4 : 8;
+  const uint64_t frame_b = p->H * p->W * in_bytes(p->in_dtype);
+  const uint64_t block_b = static_cast<uint64_t>(p->Tb) * frame_b + align256(p->nxh * p->H * p->Tb * csz);
+  const uint64_t row_out = p->nxh * (p->nl + 2) * osz;
+  const uint64_t edge_in = p->edge ? align256(p->edge_frames * frame_b) : 0;
+  const uint64_t fixed = align256(frame_b) + p->bytes_S0 + p->bytes_flags + p->bytes_long + edge_in +
+                         p->bytes_edge_ws + 8 * 256;
+  for (int mode = 1; mode <= 2; ++mode) {
+    const uint64_t fx = fixed + (mode == 1 ? p->bytes_S : 0);
+    if (fx >= budget) continue;
+    const uint64_t avail = budget - fx;
+    const uint64_t per_block = block_b + (mode == 2 ? p->nq * p->Tb * csz : 0);
+    const uint64_t per_row = row_out + (mode == 2 ? p->nxh * p->Lp * csz : 0);
+    const uint64_t nb = std::min<uint64_t>(avail / per_block, p->nblocks);
+    const uint64_t rows = std::min<uint64_t>(avail / per_row, p->ny);
+    if (nb == 0 || rows == 0) continue;
+    cp->mode = mode;
+    cp->frames_per = nb * p->Tb;
+    cp->rows_per = rows;
+    cp->fixed = fx;
+    cp->work = std::max(nb * per_block, rows * per_row) + 4 * 256;
+    return true;
+  }
+  return false;
+}
+
+RowArgs row_args(const fddm_plan* p) {
+  RowArgs ra{};
+  ra.in_dtype = p->in_dtype;
+  ra.H = static_cast<int>(p->H);
+  ra.W = static_cast<int>(p->W);
+  ra.nx = static_cast<int>(p->nx);
+  ra.Tb = p->Tb;
+  ra.G = p->Gx;
+  ra.window = p->d_window;
+  ra.logn = p->logn_x;
+  ra.norm = 1.0 / std::sqrt(static_cast<double>(p->nx) * static_cast<double>(p->ny));
+  ra.dx = p->dx;
+  ra.twx = p->d_twx;
+  return ra;
+}
+
+ColArgs col_args(const fddm_plan* p) {
+  ColArgs ca{};
+  ca.H = static_cast<int>(p->H);
+  ca.ny = static_cast<int>(p->ny);
+  ca.nxh = static_cast<int>(p->nxh);
+  ca.Tb = p->Tb;
+  ca.G = p->Gy;
+  ca.logn = p->logn_y;
+  ca.dy = p->dy;
+  ca.twy = p->d_twy;
+  return ca;
+}
+
+// Rows and columns of `frames` frames at img (frame I0 at img0 subtracted when
+// sub), blocks of Tb frames into S_dst[q][t] with row pitch `pitch`.
+int spatial_frames(const fddm_plan* p, const void* img, const void* img0, uint64_t frames, int sub, void* R,
+                   void* S_dst, uint64_t pitch, hipStream_t s, char* err, size_t errlen) {
+  RowArgs ra = row_args(p);
+  ra.img = img;
+  ra.img0 = img0;
+  ra.L = frames;
+  ra.frames = frames;
+  ra.subtract0 = sub;
+  ra.R = R;
+  FDDM_HIP(ra.logn ? launch_rows_p2(ra, p->precision, s) : launch_rows(ra, p->precision, s));
+  ColArgs ca = col_args(p);
+  ca.R = R;
+  ca.nblocks = (frames + p->Tb - 1) / p->Tb;
+  ca.Lp = pitch;
+  ca.S = S_dst;
+  FDDM_HIP(ca.logn ? launch_cols_p2(ca, p->precision, s) : launch_cols(ca, p->precision, s));
+  return FDDM_OK;
+}
+
+// Time stage and corrections of one launch range (ta.c0, ta.cols, ...).
+int time_launch(const fddm_plan* p, const TimeArgs& ta, void* lng, const void* edge_S64, hipStream_t s, char* err,
+                size_t errlen) {
+  if (p->mode == MODE_FFT && p->long_t) {
+    FDDM_HIP(launch_time_long(ta, p->logn_t, p->precision, p->out_dtype, lng, p->long_slot, p->long_slots, s));
+  } else if (p->mode == MODE_FFT) {
+    FDDM_HIP(launch_time_p2(ta, p->logn_t, p->precision, p->out_dtype, s));
+  } else {
+    FDDM_HIP(launch_diff(ta, p->precision, p->out_dtype, s));
+  }
+  if (ta.flags) FDDM_HIP(launch_wk_fix(ta, p->out_dtype, s));
+  if (p->edge) FDDM_HIP(launch_edge_fix(ta, edge_S64, p->edge->Lp, kEdgeFrames, p->edge_i0, p->out_dtype, s));
+  return FDDM_OK;
+}
+
+int host_path_chunked(fddm_plan* p, HostPathCache& c, const ChunkPlan& cp, const void* img, void* out,
+                      uint64_t* stats, char* err, size_t errlen) {
+  const hipStream_t s = c.stream;
+  const uint64_t csz = p->precision == PREC_FP64 ? 16 : 8;
+  const uint64_t osz = p->out_dtype == OUT_F32 ? 4 : 8;
+  const uint64_t frame_b = p->H * p->W * in_bytes(p->in_dtype);
+  unsigned char* dev = nullptr;
+  FDDM_HIP(hipMalloc(reinterpret_cast<void**>(&dev), cp.fixed + cp.work));
+  struct Free {
+    unsigned char* d;
+    ~Free() { (void)hipFree(d); }
+  } guard{dev};
+  // fixed pieces, then the spectra (mode 1), then the reused work region
+  unsigned char* at = dev;
+  auto take = [&](uint64_t b) {
+    unsigned char* r = at;
+    at += align256(b);
+    return r;
+  };
+  unsigned char* frame0 = take(frame_b);
+  unsigned char* S0 = take(p->bytes_S0);
+  unsigned char* flags = take(p->bytes_flags);
+  unsigned char* lng = take(p->bytes_long);
+  unsigned char* ein = take(p->edge ? p->edge_frames * frame_b : 0);
+  unsigned char* ews = take(p->bytes_edge_ws);
+  unsigned char* S = cp.mode == 1 ? take(p->bytes_S) : nullptr;
+  unsigned char* work = at;
+  std::vector<unsigned char> hostS;
+  if (cp.mode == 2) hostS.resize(p->nq * p->Lp * csz);
+  const int sub = p->subtract0 ? 1 : 0;
+  int rc;
+
+  // frame 0 (the subtracted frame) and its raw spectrum F(0) for PS
+  FDDM_HIP(hipMemcpyAsync(frame0, img, frame_b, hipMemcpyHostToDevice, s));
+  if (sub && (rc = spatial_frames(p, frame0, frame0, 1, 0, work, S0, p->Tb, s, err, errlen))) return rc;
+  // spatial stage in chunks of whole frame blocks
+  uint64_t nchunks = 0;
+  for (uint64_t t0 = 0; t0 < p->L; t0 += cp.frames_per, ++nchunks) {
+    const uint64_t fr = std::min<uint64_t>(cp.frames_per, p->L - t0);
+    unsigned char* cin = work;
+    unsigned char* R = work + align256(cp.frames_per * frame_b);
+    if ((rc = h2d_staged(c, cin, static_cast<const unsigned char*>(img) + t0 * frame_b, fr * frame_b, err, errlen)))
+      return rc;
+    if (cp.mode == 1) {
+      rc = spatial_frames(p, cin, frame0, fr, sub, R, S + t0 * csz, p->Lp, s, err, errlen);
+    } else {
+      unsigned char* piece = R + align256(cp.frames_per / p->Tb * p->nxh * p->H * p->Tb * csz);
+      rc = spatial_frames(p, cin, frame0, fr, sub, R, piece, cp.frames_per, s, err, errlen);
+      if (!rc) {
+        const uint64_t width = (fr + p->Tb - 1) / p->Tb * p->Tb * csz;
+        FDDM_HIP(hipMemcpy2DAsync(hostS.data() + t0 * csz, p->Lp * csz, piece, cp.frames_per * csz, width, p->nq,
+                                  hipMemcpyDeviceToHost, s));
+      }
+    }
+    if (rc) return rc;
+  }
+  // FP64 spectra of the edge frames (compensated path)
+  if (p->edge) {
+    // the first and last kEdgeFrames frames (every frame of a short stack)
+    const uint64_t E = kEdgeFrames;
+    if (p->L > 2 * E) {
+      FDDM_HIP(hipMemcpyAsync(ein, img, E * frame_b, hipMemcpyHostToDevice, s));
+      FDDM_HIP(hipMemcpyAsync(ein + E * frame_b, static_cast<const unsigned char*>(img) + (p->L - E) * frame_b,
+                              E * frame_b, hipMemcpyHostToDevice, s));
+    } else {
+      FDDM_HIP(hipMemcpyAsync(ein, img, p->L * frame_b, hipMemcpyHostToDevice, s));
+    }
+    const int cap = p->edge->timing_cap;
+    p->edge->timing_cap = 0;
+    rc = run_spatial(p->edge, ein, ews, s, err, errlen);
+    p->edge->timing_cap = cap;
+    if (rc) return rc;
+  }
+  FDDM_HIP(hipStreamSynchronize(s));  // the work region is reused by the time chunks
+
+  // time stage in chunks of whole ky rows, local (unshifted) pieces
+  TimeArgs ta{};
+  ta.Lp = p->Lp;
+  ta.L = static_cast<int>(p->L);
+  ta.ny = static_cast<int>(p->ny);
+  ta.nxh = static_cast<int>(p->nxh);
+  ta.S0 = sub ? S0 : nullptr;
+  ta.s0_stride = static_cast<uint64_t>(p->Tb);
+  ta.lags = static_cast<const uint32_t*>(p->d_lags);
+  ta.nl = static_cast<int>(p->nl);
+  ta.dt = p->dt;
+  ta.twt = p->d_twt;
+  ta.flags = p->bytes_flags ? flags : nullptr;
+  ta.local = 1;
+  const void* S64 = p->edge ? ews + p->edge->bytes_R : nullptr;
+  const uint64_t ny = p->ny, nxh = p->nxh, kyw = ny - ny / 2;  // rows ky >= kyw wrap to the top
+  uint64_t qchunks = 0;
+  for (uint64_t ky0 = 0; ky0 < ny; ky0 += cp.rows_per, ++qchunks) {
+    const uint64_t ky1 = std::min(ny, ky0 + cp.rows_per), cols = (ky1 - ky0) * nxh;
+    unsigned char* opiece = work;
+    if (cp.mode == 2) {
+      unsigned char* Sc = work + align256(cp.rows_per * nxh * (p->nl + 2) * osz);
+      FDDM_HIP(hipMemcpyAsync(Sc, hostS.data() + ky0 * nxh * p->Lp * csz, cols * p->Lp * csz,
+                              hipMemcpyHostToDevice, s));
+      ta.S = Sc;
+      ta.s_q0 = static_cast<int>(ky0 * nxh);
+    } else {
+      ta.S = S;
+      ta.s_q0 = 0;
+    }
+    ta.c0 = static_cast<int>(ky0 * nxh);
+    ta.cols = static_cast<int>(cols);
+    ta.ldo = cols;
+    ta.out = opiece;
+    if ((rc = time_launch(p, ta, lng, S64, s, err, errlen))) return rc;
+    // rows ky -> output rows (ky + ny/2) mod ny: at most two contiguous runs
+    for (int part = 0; part < 2; ++part) {
+      const uint64_t a0 = part == 0 ? ky0 : std::max(ky0, kyw), a1 = part == 0 ? std::min(ky1, kyw) : ky1;
+      if (a0 >= a1) continue;
+      const uint64_t yo = part == 0 ? a0 + ny / 2 : a0 - kyw;
+      FDDM_HIP(hipMemcpy2DAsync(static_cast<unsigned char*>(out) + yo * nxh * osz, p->nq * osz,
+                                opiece + (a0 - ky0) * nxh * osz, cols * osz, (a1 - a0) * nxh * osz, p->nl + 2,
+                                hipMemcpyDeviceToHost, s));
+    }
+    FDDM_HIP(hipStreamSynchronize(s));  // the piece is rewritten by the next chunk
+  }
+  stats[0] = static_cast<uint64_t>(cp.mode);
+  stats[1] = nchunks;
+  stats[2] = qchunks;
   return FDDM_OK;
 }
 
@@ -593,21 +914,46 @@ int host_path(int mode, const void* img, int dtype, uint64_t L, uint64_t H, uint
   } guard{p};
   const uint64_t img_bytes = L * H * W * in_bytes(dtype);
   const uint64_t out_b = output_bytes(p);
-  std::lock_guard<std::mutex> lk(g_cache_mu);
+  std::lock_guard<std::mutex> lk(g_cache_mu[dev]);
   HostPathCache& c = g_cache[dev];
+  if (!c.stream) FDDM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  for (int k = 0; k < 2; ++k) {
+    if (!c.pin[k]) FDDM_HIP(hipHostMalloc(&c.pin[k], kStageBytes, hipHostMallocDefault));
+    if (!c.ev[k]) FDDM_HIP(hipEventCreateWithFlags(&c.ev[k], hipEventDisableTiming));
+  }
   size_t free_b = 0, total_b = 0;
   FDDM_HIP(hipMemGetInfo(&free_b, &total_b));
+  // the device memory this call may use: the caller's budget, else 95 % of
+  // what is free plus what the cache already holds
+  const uint64_t cached = c.img_cap + c.out_cap + c.ws_cap;
+  const uint64_t budget = g_budget[dev] ? g_budget[dev]
+                                        : static_cast<uint64_t>(0.95 * static_cast<double>(free_b + cached));
+  if (img_bytes + out_b + p->ws_bytes > budget) {
+    // in pieces: drop the cached buffers first so the pieces fit beside the rest
+    FDDM_HIP(hipStreamSynchronize(c.stream));
+    for (void** ptr : {&c.img, &c.out, &c.ws})
+      if (*ptr) (void)hipFree(*ptr);
+    c.img = c.out = c.ws = nullptr;
+    c.img_cap = c.out_cap = c.ws_cap = 0;
+    FDDM_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t avail = std::min<uint64_t>(budget, static_cast<uint64_t>(0.95 * static_cast<double>(free_b)));
+    ChunkPlan cp;
+    if (!plan_chunks(p, avail, &cp))
+      return set_err(err, errlen, FDDM_ERR_MEMORY,
+                     "Not enough space on GPU: %llu bytes available, even one frame block and one row of q "
+                     "need more",
+                     static_cast<unsigned long long>(avail));
+    return host_path_chunked(p, c, cp, img, out, g_last_stats[dev], err, errlen);
+  }
   const uint64_t need = (c.img_cap >= img_bytes ? 0 : img_bytes) + (c.out_cap >= out_b ? 0 : out_b) +
                         (c.ws_cap >= p->ws_bytes ? 0 : p->ws_bytes);
   if (need > static_cast<uint64_t>(0.95 * static_cast<double>(free_b)))
     return set_err(err, errlen, FDDM_ERR_MEMORY,
                    "Not enough space on GPU: %llu bytes needed, %llu free",
                    static_cast<unsigned long long>(need), static_cast<unsigned long long>(free_b));
-  if (!c.stream) FDDM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-  for (int k = 0; k < 2; ++k) {
-    if (!c.pin[k]) FDDM_HIP(hipHostMalloc(&c.pin[k], kStageBytes, hipHostMallocDefault));
-    if (!c.ev[k]) FDDM_HIP(hipEventCreateWithFlags(&c.ev[k], hipEventDisableTiming));
-  }
+  g_last_stats[dev][0] = 0;
+  g_last_stats[dev][1] = 1;
+  g_last_stats[dev][2] = 1;
   if ((rc = grow(&c.img, &c.img_cap, img_bytes, err, errlen))) return rc;
   if ((rc = grow(&c.out, &c.out_cap, out_b, err, errlen))) return rc;
   if ((rc = grow(&c.ws, &c.ws_cap, p->ws_bytes, err, errlen))) return rc;
@@ -715,10 +1061,10 @@ int fddm_ddm_diff(const void* img, int dtype, uint64_t length, uint64_t height, 
 }
 
 int fddm_release_cache(char* err, size_t errlen) {
-  std::lock_guard<std::mutex> lk(g_cache_mu);
   int dev = 0;
   FDDM_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) return FDDM_OK;
+  std::lock_guard<std::mutex> lk(g_cache_mu[dev]);
   HostPathCache& c = g_cache[dev];
   if (c.stream) FDDM_HIP(hipStreamSynchronize(c.stream));
   for (void* ptr : {c.img, c.out, c.ws})
@@ -730,6 +1076,25 @@ int fddm_release_cache(char* err, size_t errlen) {
   if (c.stream) (void)hipStreamDestroy(c.stream);
   c = HostPathCache{};
   azavg_release_scratch(dev);
+  return FDDM_OK;
+}
+
+int fddm_set_memory_budget(uint64_t bytes, char* err, size_t errlen) {
+  int dev = 0;
+  FDDM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(err, errlen, FDDM_ERR_DEVICE, "device id %d not supported", dev);
+  std::lock_guard<std::mutex> lk(g_cache_mu[dev]);
+  g_budget[dev] = bytes;
+  return FDDM_OK;
+}
+
+int fddm_get_last_call_chunks(uint64_t* stats, char* err, size_t errlen) {
+  if (!stats) return set_err(err, errlen, FDDM_ERR_ARG, "NULL stats");
+  int dev = 0;
+  FDDM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(err, errlen, FDDM_ERR_DEVICE, "device id %d not supported", dev);
+  std::lock_guard<std::mutex> lk(g_cache_mu[dev]);
+  for (int k = 0; k < 3; ++k) stats[k] = g_last_stats[dev][k];
   return FDDM_OK;
 }
 
@@ -778,12 +1143,12 @@ int fddm_plan_timing_begin(fddm_plan* plan, int max_steps, char* err, size_t err
 
 int fddm_plan_timing_end(fddm_plan* plan, double* stage_ms, int* steps, char* err, size_t errlen) {
   if (!plan || !stage_ms || !steps) return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument");
-  for (int k = 0; k < 4; ++k) stage_ms[k] = 0.0;
+  for (int k = 0; k < kStages; ++k) stage_ms[k] = 0.0;
   *steps = plan->timing_steps;
   for (int i = 0; i < plan->timing_steps; ++i) {
     hipEvent_t* ev = &plan->events[static_cast<size_t>(i) * kEventsPerStep];
     FDDM_HIP(hipEventSynchronize(ev[kEventsPerStep - 1]));
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kStages; ++k) {
       float ms = 0.f;
       FDDM_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
       stage_ms[k] += ms;

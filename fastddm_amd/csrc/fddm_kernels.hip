@@ -39,22 +39,23 @@ int fft_bucket(long long work) {
 // arithmetic before the conversion to T.
 // ---------------------------------------------------------------------------
 template <typename Tin, typename T>
-__device__ __forceinline__ T load_pixel(const Tin* __restrict__ img, size_t idx, size_t idx0, bool sub) {
+__device__ __forceinline__ T load_pixel(const Tin* __restrict__ img, const Tin* __restrict__ ref, size_t idx, size_t idx0,
+                                        bool sub) {
   if constexpr (std::is_integral_v<Tin> && sizeof(Tin) <= 2) {
     int v = static_cast<int>(img[idx]);
-    if (sub) v -= static_cast<int>(img[idx0]);
+    if (sub) v -= static_cast<int>(ref[idx0]);
     return static_cast<T>(v);
   } else if constexpr (std::is_integral_v<Tin> && sizeof(Tin) == 4) {
     long long v = static_cast<long long>(img[idx]);
-    if (sub) v -= static_cast<long long>(img[idx0]);
+    if (sub) v -= static_cast<long long>(ref[idx0]);
     return static_cast<T>(v);
   } else if constexpr (std::is_integral_v<Tin>) {
     double v = static_cast<double>(img[idx]);
-    if (sub) v -= static_cast<double>(img[idx0]);
+    if (sub) v -= static_cast<double>(ref[idx0]);
     return static_cast<T>(v);
   } else {
     Tin v = img[idx];
-    if (sub) v -= img[idx0];
+    if (sub) v -= ref[idx0];
     return static_cast<T>(v);
   }
 }
@@ -74,6 +75,7 @@ __global__ void __launch_bounds__(kThreads) k_rows(RowArgs a) {
   const uint64_t tb = blockIdx.x / H;
   const uint64_t t0 = tb * Tb;
   const Tin* __restrict__ img = static_cast<const Tin*>(a.img);
+  const Tin* __restrict__ ref = static_cast<const Tin*>(a.img0);  // the subtracted frame
   const T* __restrict__ win = static_cast<const T*>(a.window);
   const T half_norm = static_cast<T>(0.5 * a.norm);
   const int bstride = nx + 1;
@@ -89,8 +91,8 @@ __global__ void __launch_bounds__(kThreads) k_rows(RowArgs a) {
       if (x < W) {
         const uint64_t ta = t0 + 2 * (g0 + p), tb2 = ta + 1;
         const size_t i0 = static_cast<size_t>(y) * W + x;
-        if (ta < a.frames) va = load_pixel<Tin, T>(img, ta * plane + i0, i0, sub);
-        if (tb2 < a.frames) vb = load_pixel<Tin, T>(img, tb2 * plane + i0, i0, sub);
+        if (ta < a.frames) va = load_pixel<Tin, T>(img, ref, ta * plane + i0, i0, sub);
+        if (tb2 < a.frames) vb = load_pixel<Tin, T>(img, ref, tb2 * plane + i0, i0, sub);
         if (win) {
           const T wv = win[i0];
           va *= wv;
@@ -148,14 +150,6 @@ __global__ void __launch_bounds__(kThreads) k_cols(ColArgs a) {
   }
 }
 
-// Output position of half-plane index q = ky * nxh + kx after the fftshift
-// along y: row (ky + ny/2) mod ny.
-__device__ __forceinline__ size_t shifted_q(int q, int ny, int nxh) {
-  const int ky = q / nxh, kx = q - ky * nxh;
-  const int oy = (ky + ny / 2) % ny;
-  return static_cast<size_t>(oy) * nxh + kx;
-}
-
 // ---------------------------------------------------------------------------
 // Stage (d)+(e): direct differences.  A wave works on QPW q at once, one per
 // segment of SEG = 64 / QPW lanes (QPW = 4: one q per 16-lane DPP row, 2: one
@@ -195,12 +189,14 @@ __global__ void __launch_bounds__(kThreads) k_diff(TimeArgs a, int wg_per_xcd, i
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int sl = lane & (SEG - 1), seg = lane / SEG;
-  const int qw = ((blockIdx.x & 7) * wg_per_xcd + (blockIdx.x >> 3)) * kWaves * QPW + wid * QPW;
-  if (qw >= nq) return;  // whole waves: nothing below synchronises the workgroup
-  const int q = qw + seg;
-  const bool valid = q < nq;
-  const int L = a.L, Lv = valid ? L : 0;  // segments past the last q run empty loops
-  const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(valid ? q : qw) * a.Lp;
+  // columns of the launch (nq of them): cw is this wave's first
+  const int cw = ((blockIdx.x & 7) * wg_per_xcd + (blockIdx.x >> 3)) * kWaves * QPW + wid * QPW;
+  if (cw >= nq) return;  // whole waves: nothing below synchronises the workgroup
+  const int c = cw + seg;
+  const bool valid = c < nq;
+  const int q = col_q(a, valid ? c : cw);
+  const int L = a.L, Lv = valid ? L : 0;  // segments past the last column run empty loops
+  const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(q - a.s_q0) * a.Lp;
   const C* ser = src;
   if constexpr (USE_LDS) {
     C* slice = reinterpret_cast<C*>(smem) + static_cast<size_t>(wid * QPW + seg) * L;
@@ -231,8 +227,8 @@ __global__ void __launch_bounds__(kThreads) k_diff(TimeArgs a, int wg_per_xcd, i
   const bool writer = valid && sl == SEG - 1;
 
   OutT* __restrict__ out = static_cast<OutT*>(a.out);
-  const size_t nqs = static_cast<size_t>(a.ny) * a.nxh;
-  const size_t oq = shifted_q(valid ? q : qw, a.ny, a.nxh);
+  const size_t nqs = a.ldo;
+  const size_t oq = col_out(a, valid ? c : cw);
   for (int i0 = 0; i0 < a.nl; i0 += kLagBatch) {
     int d[kLagBatch];
 #pragma unroll
@@ -374,7 +370,7 @@ namespace {
 // workgroup's four series fit 64 KiB, else read from L2.
 template <typename T, typename OutT, int PREC>
 hipError_t diff_l(const TimeArgs& a, hipStream_t s) {
-  const int nq = a.ny * a.nxh;
+  const int nq = a.cols;
   const size_t series = sizeof(cplx_t<T>) * a.L;
   constexpr size_t kFull = 160 * 1024 / 8;  // LDS per workgroup at 8 workgroups per CU
   auto go = [&](auto kernel, int qpw, size_t lds) {

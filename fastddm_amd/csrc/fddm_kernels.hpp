@@ -1,6 +1,8 @@
 // Host-side launchers of the fastddm_amd kernels.
 #pragma once
 
+#include <vector>
+
 #include "fddm_common.hpp"
 
 namespace fddm {
@@ -8,10 +10,13 @@ namespace fddm {
 // Stage (a), row pass.  Frames [0, L) of the (L, H, W) stack `img` (pixel
 // type in_dtype) -> per-row half spectra along x, scaled by `norm`, stored in
 // the blocked layout R[tb][kx][y][ti] (t = tb * Tb + ti, kx < nx/2+1, y < H).
-// `window` (H*W of T) may be null.  subtract0: use I(t) - I(0) in place of I(t).
-// `frames` limits how many leading frames are transformed (>= 1).
+// `window` (H*W of T) may be null.  subtract0: use I(t) - I0 in place of I(t),
+// where I0 is the frame at img0 (the stack's frame 0; img itself when the
+// launch starts at frame 0).  `frames` limits how many leading frames of img
+// are transformed (>= 1).
 struct RowArgs {
   const void* img;
+  const void* img0;
   int in_dtype;
   uint64_t L;       // frames present in img
   uint64_t frames;  // frames to transform (<= L)
@@ -51,7 +56,34 @@ struct TimeArgs {
   FftDesc dt;
   const void* twt;
   void* out;
+  unsigned char* flags;  // fft mode, compensated FP32: per q, 1 = some lag needs launch_wk_fix
+  // Columns of this launch, c in [0, cols).  local = 0: column c is output
+  // column c0 + c of the y-shifted plane (row pitch ldo) and reads the q at
+  // that position; local = 1: column c reads q = c0 + c and is stored
+  // unshifted at out[i ldo + c].  S holds the rows of q >= s_q0 (row q - s_q0).
+  int c0, cols, local;
+  uint64_t ldo;
+  int s_q0;
 };
+
+// Whole-plane launch: every q, y-shifted output of row pitch ny (nx/2+1).
+inline void time_full(TimeArgs& a) {
+  a.c0 = 0;
+  a.cols = a.ny * a.nxh;
+  a.local = 0;
+  a.ldo = static_cast<uint64_t>(a.ny) * a.nxh;
+  a.s_q0 = 0;
+}
+
+// q read by column c of a launch, and the column's output offset in a row.
+__device__ __forceinline__ int col_q(const TimeArgs& a, int c) {
+  if (a.local) return a.c0 + c;
+  const int o = a.c0 + c, yo = o / a.nxh, kx = o - yo * a.nxh;
+  return ((yo + a.ny - a.ny / 2) % a.ny) * a.nxh + kx;
+}
+__device__ __forceinline__ size_t col_out(const TimeArgs& a, int c) {
+  return static_cast<size_t>(a.local ? c : a.c0 + c);
+}
 
 hipError_t launch_rows(const RowArgs& a, int precision, hipStream_t s);
 hipError_t launch_cols(const ColArgs& a, int precision, hipStream_t s);
@@ -66,6 +98,33 @@ hipError_t launch_rows_p2(const RowArgs& a, int precision, hipStream_t s);
 hipError_t launch_cols_p2(const ColArgs& a, int precision, hipStream_t s);
 hipError_t launch_fft_test(void* data, int n, int batch, const FftDesc& d, const void* tw,
                            int precision, hipStream_t s);
+
+// FP64 edge correction of the compensated path (fddm_edge.hip): frames
+// [0, E) and [L - E, L) of an (L, frame_bytes) stack copied to dst, and the
+// lags lags[i0 .. nl) (n = L - dt <= E pairs) recomputed by direct sums from
+// the FP64 spectra S64[q][c] (pitch lp) of those frames into the y-shifted
+// output.
+hipError_t launch_gather_edges(const void* img, void* dst, uint64_t frame_bytes, uint64_t L, uint64_t E,
+                               hipStream_t s);
+hipError_t launch_edge_fix(const TimeArgs& a, const void* S64, uint64_t lp, int E, int i0, int out_dtype,
+                           hipStream_t s);
+constexpr int kEdgeFrames = 8;
+// Lags of the q marked in flags whose (L - dt) D < kWkFlag L VAR and
+// L - dt > 32 recomputed by direct differences of the float series S[q][t]
+// (pitch Lp) in double, written to the y-shifted output.
+constexpr double kWkFlag = 2e-3;
+hipError_t launch_wk_fix(const TimeArgs& a, int out_dtype, hipStream_t s);
+
+// Time stage for series longer than the LDS-resident plans allow
+// (fddm_time_long.hip): N = 2^logn in (2^12, 2^long_max_log] as a four-step
+// transform, one workgroup per q at a time, each with a scratch slot of
+// long_slot_bytes in `scratch` (slots slots).  long_twiddles: the table the
+// kernel reads from a.twt (complex, converted to the compute type).
+int long_max_log(int precision);
+uint64_t long_slot_bytes(int logn, int precision, uint64_t L);
+std::vector<double> long_twiddles(int logn);
+hipError_t launch_time_long(const TimeArgs& a, int logn, int precision, int out_dtype, void* scratch,
+                            uint64_t slot_bytes, int slots, hipStream_t s);
 
 // Elements-per-thread template bucket for a transform of `work` points
 // cooperatively done by 256 threads; 0 if too large.

@@ -79,7 +79,8 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems
   const size_t plane = static_cast<size_t>(H) * W;
   const bool sub = a.subtract0 != 0;
   const C* __restrict__ tw = static_cast<const C*>(a.twx);
-  const Tin* __restrict__ row0 = img + static_cast<size_t>(y) * W;  // frame 0, row y
+  const Tin* __restrict__ row0 = img + static_cast<size_t>(y) * W;  // first frame of the launch, row y
+  const Tin* __restrict__ ref0 = static_cast<const Tin*>(a.img0) + static_cast<size_t>(y) * W;  // subtracted frame
 
   // pairs (2p, 2p+1) -> LDS, element (p, x) at logical p NX + x
   if (W == NX && (static_cast<size_t>(W) * sizeof(Tin)) % 8 == 0) {
@@ -94,7 +95,7 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems
       Vec va{}, vb{}, v0{};
       if (ta < a.frames) va = *reinterpret_cast<const Vec*>(row0 + ta * plane + x0);
       if (ta + 1 < a.frames) vb = *reinterpret_cast<const Vec*>(row0 + (ta + 1) * plane + x0);
-      if (sub) v0 = *reinterpret_cast<const Vec*>(row0 + x0);
+      if (sub) v0 = *reinterpret_cast<const Vec*>(ref0 + x0);
       C* d = buf + padi(p * NX + x0);  // VX divides 32: the VX slots are contiguous
 #pragma unroll
       for (int v = 0; v < VX; ++v) {
@@ -111,7 +112,7 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems
       T fa = 0, fb = 0;
       if (x < W) {
         const T w = win ? win[static_cast<size_t>(y) * W + x] : T(1);
-        const Tin x0v = sub ? row0[x] : Tin(0);
+        const Tin x0v = sub ? ref0[x] : Tin(0);
         if (ta < a.frames) fa = pixel_value<Tin, T>(row0[ta * plane + x], x0v, sub, w);
         if (ta + 1 < a.frames) fb = pixel_value<Tin, T>(row0[(ta + 1) * plane + x], x0v, sub, w);
       }
@@ -175,12 +176,13 @@ __global__ void __launch_bounds__(64 * NP, 8) k_rows_p2_pf(RowArgs a, uint32_t n
     const int y = static_cast<int>(item % H);
     const uint64_t ta = static_cast<uint64_t>(item / H) * TB + 2 * p;
     const Tin* __restrict__ row0 = img + static_cast<size_t>(y) * NX;
+    const Tin* __restrict__ ref0 = static_cast<const Tin*>(a.img0) + static_cast<size_t>(y) * NX;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int x0 = (lane + 64 * k) * VX;
       va[k] = ta < a.frames ? *reinterpret_cast<const Vec*>(row0 + ta * plane + x0) : Vec{};
       vb[k] = ta + 1 < a.frames ? *reinterpret_cast<const Vec*>(row0 + (ta + 1) * plane + x0) : Vec{};
-      v0[k] = sub ? *reinterpret_cast<const Vec*>(row0 + x0) : Vec{};
+      v0[k] = sub ? *reinterpret_cast<const Vec*>(ref0 + x0) : Vec{};
     }
   };
 

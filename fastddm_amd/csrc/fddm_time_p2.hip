@@ -45,6 +45,13 @@ using namespace p2;
 
 constexpr int kMinThreads = 256;  // workgroup size for plans of fewer threads (G q per group)
 constexpr int kEdge = 32;
+// Compensated FP32: the float transforms leave an absolute error of at most
+// about 3e-7 x sum_t |G|^2 in (L - dt) D (measured worst case over 2 x 10^9
+// cells at N = 8192).  A q with a lag whose (L - dt) D falls below
+// kWkFlag x sum |G|^2 (2e-3: a relative error bound above ~1.5e-4) -- slow
+// low-q modes at short lags, chance near-recurrences at large lags -- is
+// marked in a.flags; launch_wk_fix then recomputes those lags by direct
+// differences.
 
 // 1 / n for a positive integer n: hardware reciprocal refined by Newton's
 // iteration r <- r (2 - n r), twice for double (error below 2 ulp).
@@ -84,18 +91,13 @@ __device__ __forceinline__ void group_sum2(double& x, double& y, double* red) {
   y = sy;
 }
 
-// q of output column o: the outputs are fftshifted along y, so column o sits
-// in row (ky + ny/2) mod ny.
-__device__ __forceinline__ int q_of_column(int o, int ny, int nxh) {
-  const int yo = o / nxh, kx = o - yo * nxh;
-  return ((yo + ny - ny / 2) % ny) * nxh + kx;
-}
-
-// Two neighbouring output values, stored as one access.
+// Two neighbouring output values, stored as one access.  A vector type keeps
+// the store whole: the store of a two-member struct is split into two scalar
+// stores, which the compiler may then merge with the unpaired fallback path.
+typedef double OutPairD __attribute__((ext_vector_type(2)));
+typedef float OutPairF __attribute__((ext_vector_type(2)));
 template <typename O>
-struct alignas(2 * sizeof(O)) OutPair {
-  O first, second;
-};
+using OutPair = std::conditional_t<sizeof(O) == 8, OutPairD, OutPairF>;
 
 // Workgroup shape and LDS layout of the time kernel at N = 2^LOGN.  Plans of
 // fewer than kMinThreads threads put G = kMinThreads / TPT q in a workgroup;
@@ -203,11 +205,11 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
   constexpr int SEQ = Lay::SEQ;
   const int wgi = (blockIdx.x & 7) * wg_per_xcd + (blockIdx.x >> 3);
   if (SEQ > 1 && SEQ * wgi >= nq) return;  // whole workgroup: before any barrier
-  const int qbase = wgi * G;
+  const int cbase = wgi * G;  // first column of this workgroup (nq: columns of the launch)
   const int L = a.L;
   const C* __restrict__ tw = static_cast<const C*>(a.twt);
   OutT* __restrict__ out = static_cast<OutT*>(a.out);
-  const size_t nqs = static_cast<size_t>(a.ny) * a.nxh;
+  const size_t nqs = a.ldo;
   const Acc inv_n = static_cast<Acc>(1) / static_cast<Acc>(N);
   Acc dsave[SEQ > 1 ? Lay::KMAX : 1];  // SEQ > 1: D of the first column per lag iteration
 
@@ -225,15 +227,14 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
     C* const buf = buf0 + g * lds_padded_size(N);
     C* const edge = edge0 + g * 2 * kEdge;
     Acc* const chunk = chunk0 + g * WPT;
-    int q;
-    if constexpr (SEQ > 1) {
-      const int o = SEQ * wgi + sq;  // output column
-      q = o < nq ? q_of_column(o, a.ny, a.nxh) : nq;
-    } else {
-      q = qbase + g;
+    const int c = SEQ > 1 ? SEQ * wgi + sq : cbase + g;  // column of the launch
+    const bool active = c < nq;
+    const int q = active ? col_q(a, c) : a.s_q0;
+    const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(q - a.s_q0) * a.Lp;
+    if constexpr (SUB && !F64) {
+      // cleared here, set after the lag loop (several barriers later) when needed
+      if (a.flags && active && u == 0) a.flags[q] = 0;
     }
-    const bool active = q < nq;
-    const C* __restrict__ src = static_cast<const C*>(a.S) + static_cast<size_t>(active ? q : 0) * a.Lp;
 
     // 1. first-pass inputs straight from HBM (clamped indices, no branches;
     // samples t >= L are replaced by the zero padding afterwards)
@@ -365,12 +366,13 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
     // lag of neighbouring q.  Lags in reverse order: the largest lags (direct
     // sums, longest loops) all fall in the first wave(s).
     {
-      const int h = tid % G, v = tid / G, qh = SEQ > 1 ? q : qbase + h;
-      if (SEQ > 1 || qh < nq) {
+      const int h = tid % G, v = tid / G, ch = SEQ > 1 ? c : cbase + h;
+      if (SEQ > 1 || ch < nq) {
         const C* const hgrp = buf0 + h * lds_padded_size(N);
         const C* const hbuf = ZA ? hgrp : hgrp + padi(NH);  // Z
         const Acc* const htail = reinterpret_cast<const Acc*>(ZA ? hgrp + padi(NH) : hgrp);
         const C* const hedge = edge0 + h * 2 * kEdge;
+        bool flag = false;  // some lag of this q needs the direct recomputation
         Acc hptot = 0;
 #pragma unroll
         for (int w = 0; w < WPT; ++w) hptot += chunk0[h * WPT + w];
@@ -416,7 +418,9 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
             const Acc ax = za.x, ay = za.y, bx = zb.x, by = zb.y;
             const Acc re = (ax + bx) + static_cast<Acc>(cs.x) * (ay + by) - static_cast<Acc>(cs.y) * (ax - bx);
             const Acc c = re * (Acc(0.5) * inv_n);
-            D = (head + tl - 2 * c) * recip<Acc>(L - d);
+            const Acc num = head + tl - 2 * c;  // (L - dt) D
+            D = num * recip<Acc>(L - d);
+            if constexpr (SUB && !F64) flag |= num * static_cast<Acc>(1.0 / kWkFlag) < hptot;
           }
           return D;
         };
@@ -424,8 +428,8 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
           // the second column stores both values (one access where the pair is
           // aligned: the first column is even, so that needs an even row
           // length and an aligned base)
-          const size_t o0 = static_cast<size_t>(SEQ) * wgi;
-          const bool vec = (nqs & 1) == 0 && reinterpret_cast<uintptr_t>(out) % sizeof(OutPair<OutT>) == 0;
+          const size_t o0 = col_out(a, SEQ * wgi);
+          const bool vec = ((nqs | o0) & 1) == 0 && reinterpret_cast<uintptr_t>(out) % sizeof(OutPair<OutT>) == 0;
           static_assert(Lay::KMAX == KL, "one lag iteration count");
 #pragma unroll
           for (int k = 0; k < KL; ++k) {
@@ -437,8 +441,7 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
             } else {
               OutT* const dst = out + static_cast<size_t>(i) * nqs + o0;
               if (active && vec) {
-                *reinterpret_cast<OutPair<OutT>*>(dst) =
-                    OutPair<OutT>{static_cast<OutT>(dsave[k]), static_cast<OutT>(D)};
+                *reinterpret_cast<OutPair<OutT>*>(dst) = OutPair<OutT>{static_cast<OutT>(dsave[k]), static_cast<OutT>(D)};
               } else {
                 dst[0] = static_cast<OutT>(dsave[k]);
                 if (active) dst[1] = static_cast<OutT>(D);
@@ -446,8 +449,7 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
             }
           }
         } else {
-          const int ky = qh / a.nxh, kx = qh - ky * a.nxh;
-          const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
+          const size_t oq = col_out(a, ch);
           if constexpr (PRE) {
 #pragma unroll
             for (int k = 0; k < KL; ++k) {
@@ -462,14 +464,16 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
             }
           }
         }
+        if constexpr (SUB && !F64) {
+          if (flag && a.flags && (SEQ == 1 || active)) a.flags[SEQ > 1 ? q : col_q(a, ch)] = 1;
+        }
       }
     }
     if (active && u == 0) {
       Acc ptot = 0;
 #pragma unroll
       for (int w = 0; w < WPT; ++w) ptot += chunk[w];
-      const int ky = q / a.nxh, kx = q - ky * a.nxh;
-      const size_t oq = static_cast<size_t>((ky + a.ny / 2) % a.ny) * a.nxh + kx;
+      const size_t oq = col_out(a, c);
       const Acc mr2 = static_cast<Acc>(fsr / L), mi2 = static_cast<Acc>(fsi / L);
       Acc ps, var;
       if constexpr (SUB) {
@@ -495,7 +499,7 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
 template <typename T, typename OutT, int LOGN, bool SUB>
 hipError_t launch_one(const TimeArgs& a, hipStream_t s) {
   using Lay = P2Lds<LOGN, T, SUB>;
-  const int nq = a.ny * a.nxh;
+  const int nq = a.cols;
   // two-column plans keep at most KMAX lags per thread in registers (nl < L <= N/2 fits)
   if (Lay::SEQ > 1 && a.nl > Lay::KMAX * Lay::NT) return hipErrorInvalidValue;
   const int wgs = (nq + Lay::G * Lay::SEQ - 1) / (Lay::G * Lay::SEQ);

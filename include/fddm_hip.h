@@ -104,6 +104,19 @@ int fddm_ddm_diff(const void *img, int dtype, uint64_t length, uint64_t height, 
 /* Drops the device buffers cached by fddm_ddm_fft / fddm_ddm_diff and the
  * scratch kept by fddm_azimuthal_average, on the current device. */
 int fddm_release_cache(char *err, size_t errlen);
+/* Device memory fddm_ddm_fft / fddm_ddm_diff may use on the current device
+ * (0: 95 % of free memory, the default).  A call whose input, workspace and
+ * output exceed it runs in pieces -- the spatial stage over chunks of frames,
+ * the time stage over chunks of q, with the q-major spectra in device memory
+ * or, when they do not fit either, in host memory -- and gives the same
+ * result bit for bit.  This replaces the reference's fixed-point planner of
+ * loop counts over frames and q (src/cuda/memchk_gpu.cu:141-702, driven by
+ * free memory x 0.9) and lets a caller cap the footprint. */
+int fddm_set_memory_budget(uint64_t bytes, char *err, size_t errlen);
+/* How the last fddm_ddm_fft / fddm_ddm_diff call on the current device ran:
+ * stats[3] = {0 one launch per stage | 1 pieces, spectra in device memory |
+ * 2 pieces, spectra in host memory, frame chunks, q chunks}. */
+int fddm_get_last_call_chunks(uint64_t *stats, char *err, size_t errlen);
 
 /* ---- plans: device-resident inputs and outputs ----
  * The phases of ddm_fft_cuda / ddm_diff_cuda (compute_fft2,
@@ -125,7 +138,8 @@ int fddm_plan_execute(fddm_plan *plan, const void *d_img, void *d_out, void *d_w
 /* Per-stage timing with HIP events on the execute stream: begin reserves
  * events for up to max_steps executes; end waits for them and writes the
  * summed milliseconds of the stages {frame-0 spectra, row pass, column pass,
- * time/diff kernel} to stage_ms[4] and the number of timed executes. */
+ * time/diff kernel, FP64 edge correction (FP32C only)} to stage_ms[5] and the
+ * number of timed executes. */
 int fddm_plan_timing_begin(fddm_plan *plan, int max_steps, char *err, size_t errlen);
 int fddm_plan_timing_end(fddm_plan *plan, double *stage_ms, int *steps, char *err, size_t errlen);
 /* Test entry: spatial stages only; copies the q-major spectra (ny*(nx/2+1),

@@ -70,8 +70,9 @@ def _plan_create(lib, **over):
     (dict(nx=4), 1, "nx, ny"),
     (dict(nx=11, W=11), 2, "2^a 3^b 5^c 7^d"),
     (dict(nt=10), 1, "nt must be"),
-    (dict(L=10000, nt=20000), 2, "too long"),
-    (dict(L=5000, nt=10000, precision=0), 2, "too long"),
+    # beyond the four-step time transform: 2^17 points (fp32), 2^16 (fp64)
+    (dict(L=70000, nt=140000, precision=1), 2, "too long"),
+    (dict(L=40000, nt=80000, precision=0), 2, "too long"),
 ])
 def test_plan_argument_errors_are_reported(lib, over, code, text):
     rc, msg = _plan_create(lib, **over)

@@ -4,7 +4,9 @@ properties that hold at any size, since the oracle cannot run there in
 seconds:
 
 * the compensated FP32 path agrees with the FP64 path to a median relative
-  error of 1e-5 (and a bounded worst case);
+  error of 1e-5, and every element within the reference's single-precision
+  tolerance, rtol 1e-3 with atol 0 (tests/test_ddm.py:116-157 of the
+  reference);
 * D is a mean of squared moduli: positive for these noisy frames, and
   PS >= VAR >= 0;
 * direct differences (diff mode) on a subset of lags reproduce the
@@ -46,7 +48,8 @@ def test_full_size_properties(hip):
     nz = ref != 0
     rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
     assert np.median(rel) <= 1e-5, np.median(rel)
-    assert rel.max() <= 1e-2, rel.max()
+    assert rel.max() <= 1e-3, rel.max()
+    assert np.array_equal(got == 0, ref == 0)
     del got, rel, nz
 
     D, PS, VAR = ref[:-2], ref[-2], ref[-1]
@@ -63,14 +66,16 @@ def test_full_size_properties(hip):
 def test_large_size_properties(hip):
     """4000 frames of 1024 x 1024 uint16 (nt = 8000: the two-column N = 8192
     time plan, 8-frame blocks whose 64-byte runs take the XCD-aware order,
-    FP64 rows on 4-frame blocks), about 80 log-spaced lags from 1 to 3999:
-    the same properties as above."""
+    FP64 rows on 4-frame blocks), about 80 log-spaced lags from 1 to 3999 and
+    the last 80: the same properties as above."""
     Lb, Hb, Wb = 4000, 1024, 1024
     print(f"large size: {Lb} x {Hb} x {Wb} uint16, log-spaced lags", flush=True)
     d_img = hip.DeviceBuffer(Lb * Hb * Wb * 2)
     hip.fill_uniform(d_img, Lb * Hb * Wb, np.uint16, 21, 800, 1200)
     sub = np.array([1, 37, 1000, Lb - 1])  # also checked by direct differences
-    lags = np.unique(np.concatenate([np.geomspace(1, Lb - 1, 100).astype(int), sub]))
+    # log-spaced, plus every lag with 1..80 frame pairs: the few-pair lags are
+    # where the float transforms' rounding is largest relative to D
+    lags = np.unique(np.concatenate([np.geomspace(1, Lb - 1, 100).astype(int), np.arange(Lb - 80, Lb), sub]))
     win = fwin.blackman_harris((Hb, Wb))
 
     def run(mode, lg, precision):
@@ -90,7 +95,8 @@ def test_large_size_properties(hip):
     nz = ref != 0
     rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
     assert np.median(rel) <= 1e-5, np.median(rel)
-    assert rel.max() <= 1e-2, rel.max()
+    assert rel.max() <= 1e-3, rel.max()
+    assert np.array_equal(got == 0, ref == 0)
     del got, rel, nz
 
     D, PS, VAR = ref[:-2], ref[-2], ref[-1]
@@ -103,3 +109,27 @@ def test_large_size_properties(hip):
     assert np.allclose(direct[:-2], D[idx], rtol=1e-9, atol=0)
     assert np.allclose(direct[-2:], ref[-2:], rtol=1e-9, atol=0)
     d_img.free()
+
+
+def test_benchmark_speckle_precision(hip):
+    """The benchmark's own input (2000 x 512^2 uint16 AR(1) speckle, sigma 1.5,
+    D 0.2, mean 1000, std 150, seed 2), all lags, Blackman-Harris: slow low-q
+    modes whose D at large lags is small against their variance -- where the
+    Wiener-Khinchin rounding of the float transforms is largest relative to
+    D -- and chance near-recurrences at the last lag.  fp32c within rtol 1e-3
+    of fp64 everywhere, median 1e-5."""
+    from fastddm_amd.speckle import CONFIGS, ar1_speckle
+
+    img = ar1_speckle(**CONFIGS["precision_512"])
+    print(f"benchmark speckle: {img.shape} {img.dtype}", flush=True)
+    d_img = hip.DeviceBuffer.from_array(img)
+    del img
+    lags = np.arange(1, L)
+    ref = _run(hip, d_img, "fft", lags, "fp64")
+    got = _run(hip, d_img, "fft", lags, "fp32c")
+    d_img.free()
+    nz = ref != 0
+    rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
+    print(f"benchmark speckle: fp32c vs fp64 median {np.median(rel):.2e} max {rel.max():.2e}", flush=True)
+    assert np.median(rel) <= 1e-5, np.median(rel)
+    assert rel.max() <= 1e-3, rel.max()

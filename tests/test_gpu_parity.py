@@ -161,7 +161,8 @@ def test_whole_path_matches_oracle(hip, mode, case, precision):
 
 # Every power-of-two time plan (N = 512 .. 16384: three- and four-pass plans,
 # 64 .. 1024 threads per transform) on a small image.
-TIME_LENGTHS = [(200, 512), (300, 1024), (700, 2048), (1500, 4096), (3000, 8192), (5000, 16384)]
+TIME_LENGTHS = [(200, 512), (300, 1024), (700, 2048), (1500, 4096), (3000, 8192), (5000, 16384),
+                (10000, 32768)]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32c", "fp32"])
@@ -172,10 +173,8 @@ def test_every_time_plan_matches_oracle(hip, L, N, precision):
     lags = np.unique(np.concatenate([np.arange(1, 40), np.geomspace(40, L - 1, 60).astype(int),
                                      np.arange(L - 40, L)]))
     nt = next_fast_len(2 * L, "hip")
-    if precision == "fp64" and L > 4096:
-        with pytest.raises(RuntimeError, match="too long"):
-            hip.ddm_fft(img, lags, 10, 12, nt, None, precision=precision)
-        return
+    # fp64 from 5000 frames and fp32 from 10000 run the four-step time
+    # transform through device scratch (N = 4096 x 4 or x 8)
     ref = onp.ddm_fft(img, lags, 10, 12, 2 * L, None, subtract_mean=True)
     out = hip.ddm_fft(img, lags, 10, 12, nt, None, precision=precision)
     CHECK[precision](out, ref)
@@ -299,7 +298,28 @@ def test_ddm_front_end_with_hip_core(hip):
         fddm.ddm(img, [1], core="hip", gpu_id=99)
 
 
+@pytest.mark.parametrize("precision,L,size", [("fp64", 5000, 64), ("fp32c", 9000, 48), ("fp32", 9000, 48),
+                                               ("fp64", 20000, 6), ("fp32c", 33000, 4)])
+def test_four_step_time_transform(hip, precision, L, size):
+    """Series beyond the LDS-resident plans (fp64 over 4096 frames, fp32 over
+    8192): the four-step transform of length 4096 x N1 (N1 = 4 .. 16) with one
+    workgroup per q; at 64 x 64 and 48 x 48 there are more q than scratch slots,
+    so workgroups loop over several q.  All lags for the short images, a
+    log-spaced set plus the last 40 for the large ones."""
+    img = ar1_speckle(L, size, size, sigma=1.2, diffusivity=0.05, mean=800, std=120, seed=L + size)
+    if size <= 8:
+        lags = np.arange(1, L)
+    else:
+        lags = np.unique(np.concatenate([np.arange(1, 40), np.geomspace(40, L - 1, 80).astype(int),
+                                         np.arange(L - 40, L)]))
+    nt = next_fast_len(2 * L, "hip")
+    ref = onp.ddm_fft(img, lags, size, size, 2 * L, None, subtract_mean=True)
+    out = hip.ddm_fft(img, lags, size, size, nt, None, precision=precision)
+    CHECK[precision](out, ref)
+
+
 def test_too_long_series_is_rejected(hip):
-    img = np.zeros((5000, 4, 4), np.uint8)
+    # beyond 2^16 points in fp64 (32768 frames)
+    img = np.zeros((40000, 4, 4), np.uint8)
     with pytest.raises(RuntimeError, match="too long"):
-        hip.ddm_fft(img, [1], 4, 4, 10000, None, precision="fp64")
+        hip.ddm_fft(img, [1], 4, 4, 80000, None, precision="fp64")
