@@ -66,6 +66,7 @@ def lib() -> ctypes.CDLL:
         "fddm_plan_workspace_size": [p, ctypes.POINTER(u64)],
         "fddm_plan_output_size": [p, ctypes.POINTER(u64)],
         "fddm_plan_execute": [p, p, p, p, p, cp, sz],
+        "fddm_plan_execute_static": [p, p, p, p, cp, sz],
         "fddm_plan_timing_begin": [p, i32, cp, sz],
         "fddm_plan_timing_end": [p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32), cp, sz],
         "fddm_plan_spectra": [p, p, p, p, cp, sz],
@@ -124,6 +125,7 @@ EXPORTED_SYMBOLS = (
     "fddm_plan_workspace_size",
     "fddm_plan_output_size",
     "fddm_plan_execute",
+    "fddm_plan_execute_static",
     "fddm_plan_timing_begin",
     "fddm_plan_timing_end",
     "fddm_plan_spectra",
@@ -465,6 +467,16 @@ class Plan:
                 stream: Optional[Stream] = None) -> None:
         self._check_buffers(d_img, d_ws, d_out)
         _call("fddm_plan_execute", self.handle, d_img.ptr, d_out.ptr, d_ws.ptr,
+              stream.handle if stream is not None else None)
+
+    STATIC_SCRATCH_BYTES = 256 << 20
+
+    def execute_static(self, d_img: DeviceBuffer, d_out: DeviceBuffer, stream: Optional[Stream] = None) -> None:
+        """``execute`` with the library's static per-device workspace
+        (256 MiB; no allocation): plans with a larger workspace are refused.
+        Calls that overlap on one device share that workspace and race."""
+        self._check_buffers(d_img, None, d_out)
+        _call("fddm_plan_execute_static", self.handle, d_img.ptr, d_out.ptr,
               stream.handle if stream is not None else None)
 
     def timing_begin(self, max_steps: int) -> None:

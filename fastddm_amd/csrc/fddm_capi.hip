@@ -964,6 +964,31 @@ int host_path(int mode, const void* img, int dtype, uint64_t L, uint64_t H, uint
   return FDDM_OK;
 }
 
+// Workspace of fddm_plan_execute_static: one array per device, held from
+// the library's load to the end of the process.  Its device address comes
+// from hipGetSymbolAddress, once per device (the array's name is only a host
+// stand-in); nothing in it has to start at zero, since every execute writes
+// each workspace element it later reads.
+constexpr uint64_t kStaticScratchBytes = uint64_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+int static_scratch(void** ws, char* err, size_t errlen) {
+  int dev = 0;
+  FDDM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(err, errlen, FDDM_ERR_DEVICE, "device id %d not supported", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    FDDM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_err(err, errlen, FDDM_ERR_HIP, "static scratch has no device address");
+    g_scratch_ptr[dev] = p;
+  }
+  *ws = g_scratch_ptr[dev];
+  return FDDM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1128,6 +1153,19 @@ int fddm_plan_output_size(const fddm_plan* plan, uint64_t* bytes) {
 int fddm_plan_execute(fddm_plan* plan, const void* d_img, void* d_out, void* d_workspace, void* stream,
                       char* err, size_t errlen) {
   return plan_execute(plan, d_img, d_out, d_workspace, static_cast<hipStream_t>(stream), err, errlen);
+}
+
+int fddm_plan_execute_static(fddm_plan* plan, const void* d_img, void* d_out, void* stream, char* err,
+                             size_t errlen) {
+  if (!plan || !d_img || !d_out) return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument to execute");
+  if (plan->ws_bytes > kStaticScratchBytes)
+    return set_err(err, errlen, FDDM_ERR_SIZE, "workspace of %llu bytes exceeds the static scratch (%llu bytes)",
+                   static_cast<unsigned long long>(plan->ws_bytes),
+                   static_cast<unsigned long long>(kStaticScratchBytes));
+  void* ws = nullptr;
+  const int rc = static_scratch(&ws, err, errlen);
+  if (rc) return rc;
+  return plan_execute(plan, d_img, d_out, ws, static_cast<hipStream_t>(stream), err, errlen);
 }
 
 int fddm_plan_timing_begin(fddm_plan* plan, int max_steps, char* err, size_t errlen) {

@@ -135,6 +135,16 @@ int fddm_plan_output_size(const fddm_plan *plan, uint64_t *bytes);
 /* d_img, d_out, d_workspace: device pointers; stream: hipStream_t or NULL. */
 int fddm_plan_execute(fddm_plan *plan, const void *d_img, void *d_out, void *d_workspace,
                       void *stream, char *err, size_t errlen);
+/* fddm_plan_execute without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  It
+ * allocates nothing: the workspace is a static device array of 256 MiB per
+ * GPU (held from library load to process exit); plans whose
+ * fddm_plan_workspace_size exceeds it are rejected with FDDM_ERR_SIZE.  All
+ * calls on one GPU share that array and it keeps the last call's contents,
+ * so calls that overlap on one GPU (other streams, other threads) race. */
+int fddm_plan_execute_static(fddm_plan *plan, const void *d_img, void *d_out, void *stream, char *err,
+                             size_t errlen);
 /* Per-stage timing with HIP events on the execute stream: begin reserves
  * events for up to max_steps executes; end waits for them and writes the
  * summed milliseconds of the stages {frame-0 spectra, row pass, column pass,
