@@ -359,10 +359,18 @@ def serve_main(args):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
 
     stacks = {}
+    host = args.serve_host_inputs
     for k, sz in enumerate(sizes):
-        b = core.DeviceBuffer(4000 * int(sz) * int(sz) * 2)
-        core.fill_uniform(b, 4000 * int(sz) * int(sz), np.uint16, 1000 + k, 800, 1200)
-        stacks[int(sz)] = b
+        sz = int(sz)
+        if host:
+            # host stacks: 64 random frames repeated (the values do not matter
+            # for latency); each request's frames cross the host link at admission
+            base = np.random.default_rng(1000 + k).integers(800, 1200, (64, sz, sz)).astype(np.uint16)
+            stacks[sz] = np.ascontiguousarray(np.tile(base, (4000 // 64 + 1, 1, 1))[:4000])
+        else:
+            b = core.DeviceBuffer(4000 * sz * sz * 2)
+            core.fill_uniform(b, 4000 * sz * sz, np.uint16, 1000 + k, 800, 1200)
+            stacks[sz] = b
     srv = core.Server(max_slots=args.serve_slots)
     rid_of = {}
     done = {}
@@ -391,14 +399,18 @@ def serve_main(args):
         sz, l = int(size[i]), int(L[i])
         lags = np.arange(1, l) if mode[i] == "fft" else logspace_int(l - 1, num=50, endpoint=True)
         nt = next_fast_len(2 * l, "hip") if mode[i] == "fft" else 0
-        rid = srv.submit(mode[i], stacks[sz], lags, sz, sz, nt, precision="fp32c", img_on_device=True,
-                         shape=(l, sz, sz), dtype=np.uint16)
+        if host:
+            rid = srv.submit(mode[i], stacks[sz][:l], lags, sz, sz, nt, precision="fp32c")
+        else:
+            rid = srv.submit(mode[i], stacks[sz], lags, sz, sz, nt, precision="fp32c", img_on_device=True,
+                             shape=(l, sz, sz), dtype=np.uint16)
         rid_of[i] = rid
     th.join(timeout=600)
     stats = srv.stats()
     srv.close()
-    for b in stacks.values():
-        b.free()
+    if not host:
+        for b in stacks.values():
+            b.free()
     if len(done) != n:
         raise RuntimeError(f"only {len(done)} of {n} requests finished")
     lat = np.empty(n)
@@ -432,8 +444,12 @@ def serve_main(args):
         "arena_peak_GB": stats["arena_peak"] / 1e9,
         "plans": stats["plans"],
         "per_size": per_size,
-        "note": ("device-resident uint16 stacks (uniform integers), fp32c, float64 results kept on "
-                 "device until released; latency includes plan creation"),
+        "inputs": "host" if host else "device",
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default"),
+        "note": (("host uint16 stacks copied to the device at admission (inside the latency)" if host else
+                  "device-resident uint16 stacks (uniform integers)") +
+                 ", fp32c, float64 results kept on device until released; latency includes plan creation; "
+                 "requests above 512 MB of spectra run as launches of 256 MB each"),
     }
 
 
@@ -576,6 +592,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=500)
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate (1/s)")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-host-inputs", action="store_true",
+                    help="serve from host stacks (the H2D of each request inside its latency)")
+    ap.add_argument("--serve-hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for the serving process (0: runtime default, at most 32)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device-to-device copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -601,9 +621,14 @@ def main():
         cmd = [sys.executable, os.path.abspath(__file__), "--serve-only",
                "--serve-requests", str(args.serve_requests), "--serve-rate", str(args.serve_rate),
                "--serve-slots", str(args.serve_slots)]
+        if args.serve_host_inputs:
+            cmd.append("--serve-host-inputs")
+        env = dict(os.environ)
+        if 0 < args.serve_hw_queues <= 32:
+            env["GPU_MAX_HW_QUEUES"] = str(args.serve_hw_queues)
         t0 = time.perf_counter()
         try:
-            res = subprocess.run(cmd, capture_output=True, text=True, timeout=420)
+            res = subprocess.run(cmd, capture_output=True, text=True, timeout=420, env=env)
             lines = [l for l in res.stdout.splitlines() if l.startswith("{")]
             serve = json.loads(lines[-1]) if res.returncode == 0 and lines else {
                 "error": f"serving run exited with {res.returncode}: {res.stderr[-400:]}"}

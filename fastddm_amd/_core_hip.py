@@ -60,6 +60,11 @@ def lib() -> ctypes.CDLL:
         "fddm_release_cache": [cp, sz],
         "fddm_set_memory_budget": [u64, cp, sz],
         "fddm_get_last_call_chunks": [ctypes.POINTER(u64), cp, sz],
+        "fddm_ingest_begin": [ctypes.POINTER(p), i32, i32, u64, u64, u64, p, u64, u64, u64, u64, p, i32, i32, i32,
+                              cp, sz],
+        "fddm_ingest_push": [p, p, u64, cp, sz],
+        "fddm_ingest_finish": [p, p, cp, sz],
+        "fddm_ingest_destroy": [p],
         "fddm_plan_create": [ctypes.POINTER(p), i32, i32, u64, u64, u64, p, u64, u64, u64, u64,
                              p, i32, i32, i32, cp, sz],
         "fddm_plan_destroy": [p],
@@ -120,6 +125,10 @@ EXPORTED_SYMBOLS = (
     "fddm_release_cache",
     "fddm_set_memory_budget",
     "fddm_get_last_call_chunks",
+    "fddm_ingest_begin",
+    "fddm_ingest_push",
+    "fddm_ingest_finish",
+    "fddm_ingest_destroy",
     "fddm_plan_create",
     "fddm_plan_destroy",
     "fddm_plan_workspace_size",
@@ -404,6 +413,54 @@ class Stream:
     def __del__(self):
         try:
             self.destroy()
+        except Exception:
+            pass
+
+
+class Ingest:
+    """Streamed analysis of one stack whose frames arrive in pieces.
+
+    ``push`` takes (k, H, W) frames in stack order (any k); their transfer and
+    row / column transforms overlap later pushes.  ``finish`` returns the
+    (NL+2, ny, nx//2+1) result once all L frames are pushed -- the values of
+    ``ddm_fft`` / ``ddm_diff`` on the whole stack.
+    """
+
+    def __init__(self, mode, dtype, shape, lags, nx, ny, nt=0, window=None, precision="fp64",
+                 out_dtype=np.float64):
+        lg = _lags_arg(lags)
+        L, H, W = shape
+        wp, wf32, _keep = _window_arg(window, H, W)
+        self.shape = tuple(int(v) for v in shape)
+        self.dtype = np.dtype(dtype)
+        self.out_shape = (len(lg) + 2, ny, nx // 2 + 1)
+        self.out_dtype = np.dtype(out_dtype)
+        self._h = ctypes.c_void_p()
+        _call("fddm_ingest_begin", ctypes.byref(self._h), MODE_FFT if mode == "fft" else MODE_DIFF,
+              DTYPE_CODES[self.dtype], L, H, W, lg.ctypes.data, len(lg), nx, ny, nt, wp, wf32,
+              _prec_code(precision), _out_code(out_dtype))
+
+    def push(self, frames: np.ndarray) -> None:
+        a = np.ascontiguousarray(frames, dtype=self.dtype)
+        if a.ndim == 2:
+            a = a[None]
+        if a.shape[1:] != self.shape[1:]:
+            raise RuntimeError(f"frames of shape {a.shape[1:]} pushed into a {self.shape[1:]} stack")
+        _call("fddm_ingest_push", self._h, a.ctypes.data, a.shape[0])
+
+    def finish(self) -> np.ndarray:
+        out = np.empty(self.out_shape, dtype=self.out_dtype)
+        _call("fddm_ingest_finish", self._h, out.ctypes.data)
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().fddm_ingest_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

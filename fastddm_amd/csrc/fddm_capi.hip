@@ -247,6 +247,7 @@ void plan_free_device(fddm_plan* p) {
 }  // namespace
 
 int fddm::plan_frames_per_block(const fddm_plan* p) { return p ? p->Tb : 0; }
+uint64_t fddm::plan_spectra_bytes(const fddm_plan* p) { return p ? p->bytes_S : 0; }
 
 // frames_per_block: 0 picks the default block size; 2, 4, 8 or 16 forces it
 // (the autotuner of fddm_batch.hip tries several).
@@ -765,6 +766,118 @@ int time_launch(const fddm_plan* p, const TimeArgs& ta, void* lng, const void* e
   if (p->edge) FDDM_HIP(launch_edge_fix(ta, edge_S64, p->edge->Lp, kEdgeFrames, p->edge_i0, p->out_dtype, s));
   return FDDM_OK;
 }
+
+}  // namespace
+
+// plan_execute in smaller launches: the spatial stage over chunks of
+// frames_per frames (whole frame blocks) into the same workspace, the time
+// stage and its corrections over chunks of cols_per output columns.  Each
+// launch is short, so kernels of other streams (other requests of the
+// server) are dispatched between them; the result is bit-identical.
+int fddm::plan_spatial_frames(fddm_plan* p, const void* img, const void* img0, uint64_t t0, uint64_t frames, void* ws,
+                              hipStream_t s, char* err, size_t errlen) {
+  const uint64_t csz = p->precision == PREC_FP64 ? 16 : 8;
+  const uint64_t rblock = p->nxh * p->H * p->Tb * csz;  // R bytes per frame block
+  auto* base = static_cast<unsigned char*>(ws);
+  return spatial_frames(p, img, img0, frames, p->subtract0 ? 1 : 0, base + t0 / p->Tb * rblock,
+                        base + p->bytes_R + t0 * csz, p->Lp, s, err, errlen);
+}
+
+PlanInfo fddm::plan_info(const fddm_plan* p) {
+  PlanInfo i{};
+  i.L = p->L;
+  i.Tb = p->Tb;
+  i.in_bytes = in_bytes(p->in_dtype);
+  i.ws_bytes = p->ws_bytes;
+  i.out_bytes = output_bytes(p);
+  i.edge_frames = p->edge ? p->edge_frames : 0;
+  return i;
+}
+
+// The stages after the spatial pass over every frame: F(0) from frame0, the
+// FP64 edge spectra from edge_in (the child's frames, or gathered from img
+// when edge_in is null), then the time stage and its corrections over the
+// whole plane in launches of cols_per columns.
+int finish_from_spectra(fddm_plan* p, const void* frame0, const void* img, const void* edge_in, void* ws,
+                        void* d_out, int cols_per, hipStream_t s, char* err, size_t errlen) {
+  auto* base = static_cast<unsigned char*>(ws);
+  unsigned char* R = base;
+  unsigned char* S = base + p->bytes_R;
+  unsigned char* S0 = S + p->bytes_S;
+  unsigned char* flags = S0 + p->bytes_S0;
+  unsigned char* lng = flags + p->bytes_flags;
+  const int nq = static_cast<int>(p->nq);
+  const int sub = p->subtract0 ? 1 : 0;
+  int rc;
+  if (sub && (rc = spatial_frames(p, frame0, frame0, 1, 0, R, S0, p->Tb, s, err, errlen))) return rc;
+  const void* S64 = nullptr;
+  if (p->edge) {
+    unsigned char* ein = base + p->ws_bytes - p->bytes_edge_ws - p->bytes_edge_in;
+    unsigned char* ews = base + p->ws_bytes - p->bytes_edge_ws;
+    const void* src = edge_in ? edge_in : img;
+    if (!edge_in && p->bytes_edge_in) {
+      FDDM_HIP(launch_gather_edges(img, ein, p->H * p->W * in_bytes(p->in_dtype), p->L, kEdgeFrames, s));
+      src = ein;
+    }
+    const int cap = p->edge->timing_cap;
+    p->edge->timing_cap = 0;
+    rc = run_spatial(p->edge, src, ews, s, err, errlen);
+    p->edge->timing_cap = cap;
+    if (rc) return rc;
+    S64 = ews + p->edge->bytes_R;
+  }
+  TimeArgs ta{};
+  ta.S = S;
+  ta.Lp = p->Lp;
+  ta.L = static_cast<int>(p->L);
+  ta.ny = static_cast<int>(p->ny);
+  ta.nxh = static_cast<int>(p->nxh);
+  ta.S0 = sub ? S0 : nullptr;
+  ta.s0_stride = static_cast<uint64_t>(p->Tb);
+  ta.lags = static_cast<const uint32_t*>(p->d_lags);
+  ta.nl = static_cast<int>(p->nl);
+  ta.dt = p->dt;
+  ta.twt = p->d_twt;
+  ta.out = d_out;
+  ta.flags = p->bytes_flags ? flags : nullptr;
+  time_full(ta);
+  if (cols_per <= 0) cols_per = nq;
+  for (int c0 = 0; c0 < nq; c0 += cols_per) {
+    ta.c0 = c0;
+    ta.cols = std::min(cols_per, nq - c0);
+    if ((rc = time_launch(p, ta, lng, S64, s, err, errlen))) return rc;
+  }
+  return FDDM_OK;
+}
+
+int fddm::plan_finish_from_spectra(fddm_plan* p, const void* frame0, const void* edge_in, void* ws, void* d_out,
+                                   hipStream_t s, char* err, size_t errlen) {
+  return finish_from_spectra(p, frame0, nullptr, edge_in, ws, d_out, 0, s, err, errlen);
+}
+
+// plan_execute in smaller launches: the spatial stage over chunks of
+// frames_per frames (whole frame blocks) into the same workspace, the time
+// stage and its corrections over chunks of cols_per output columns.  Each
+// launch is short, so kernels of other streams (other requests of the
+// server) are dispatched between them; the result is bit-identical.
+int fddm::plan_execute_pieces(fddm_plan* p, const void* d_img, void* d_out, void* ws, hipStream_t s,
+                              uint64_t frames_per, int cols_per, char* err, size_t errlen) {
+  if (!p || !d_img || !d_out || !ws) return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument to execute");
+  const int nq = static_cast<int>(p->nq);
+  if ((frames_per == 0 || frames_per >= p->L) && (cols_per <= 0 || cols_per >= nq))
+    return plan_execute(p, d_img, d_out, ws, s, err, errlen);
+  frames_per = frames_per == 0 ? p->L : std::max<uint64_t>(p->Tb, frames_per / p->Tb * p->Tb);
+  const uint64_t frame_b = p->H * p->W * in_bytes(p->in_dtype);
+  const auto* img = static_cast<const unsigned char*>(d_img);
+  int rc;
+  for (uint64_t t0 = 0; t0 < p->L; t0 += frames_per) {
+    const uint64_t fr = std::min<uint64_t>(frames_per, p->L - t0);
+    if ((rc = plan_spatial_frames(p, img + t0 * frame_b, img, t0, fr, ws, s, err, errlen))) return rc;
+  }
+  return finish_from_spectra(p, img, img, nullptr, ws, d_out, cols_per, s, err, errlen);
+}
+
+namespace {
 
 int host_path_chunked(fddm_plan* p, HostPathCache& c, const ChunkPlan& cp, const void* img, void* out,
                       uint64_t* stats, char* err, size_t errlen) {

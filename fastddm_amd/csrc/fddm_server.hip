@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fddm_hip.h"
+#include "fddm_plan.hpp"
 
 namespace {
 
@@ -64,6 +65,7 @@ size_t pixel_bytes(int dtype) {
 }
 
 constexpr uint64_t kAlign = 256;
+constexpr uint64_t kPieceBytes = uint64_t(256) << 20;  // spectra per launch of a large request
 uint64_t align_up(uint64_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
 
 // First-fit allocator over one device range; blocks are kept sorted by
@@ -244,8 +246,21 @@ struct fddm_server {
           rc = set_err(err, sizeof err, FDDM_ERR_HIP, "input copy failed");
         img = base + r->in_off;
       }
-      // the plan was created at submission; execute only
-      if (!rc) rc = fddm_plan_execute(r->plan, img, base + r->out_off, base + r->ws_off, s, err, sizeof err);
+      // the plan was created at submission; execute only.  Large requests run
+      // as launches of about kPieceBytes of spectra each, so the kernels of
+      // requests on other slots are dispatched between them.
+      if (!rc) {
+        const uint64_t sb = fddm::plan_spectra_bytes(r->plan);
+        if (sb > 2 * kPieceBytes) {
+          const uint64_t L = r->key.L, nq = r->key.ny * (r->key.nx / 2 + 1);
+          const uint64_t frames = std::max<uint64_t>(1, L * kPieceBytes / sb);
+          const int cols = static_cast<int>(std::max<uint64_t>(256, nq * kPieceBytes / sb));
+          rc = fddm::plan_execute_pieces(r->plan, img, base + r->out_off, base + r->ws_off, s, frames, cols, err,
+                                         sizeof err);
+        } else {
+          rc = fddm_plan_execute(r->plan, img, base + r->out_off, base + r->ws_off, s, err, sizeof err);
+        }
+      }
       if (hipStreamSynchronize(s) != hipSuccess && !rc)
         rc = set_err(err, sizeof err, FDDM_ERR_HIP, "stream synchronisation failed");
       {

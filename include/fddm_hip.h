@@ -118,6 +118,27 @@ int fddm_set_memory_budget(uint64_t bytes, char *err, size_t errlen);
  * 2 pieces, spectra in host memory, frame chunks, q chunks}. */
 int fddm_get_last_call_chunks(uint64_t *stats, char *err, size_t errlen);
 
+/* ---- streamed ingest: frames pushed piece by piece ----
+ * For callers that read or decode the stack while it is analysed (a TIFF
+ * reader, a camera): replaces the reference's read-all-then-analyse flow
+ * (src/python/utils.py:23-257 loading the whole stack, then the frame-batch
+ * H2D of compute_fft2, src/cuda/ddm_cuda.cu:40-261).  Pushed frames are
+ * staged in pinned host memory in chunks of whole frame blocks; each chunk's
+ * DMA (copy stream) and row and column passes (compute stream) overlap the
+ * caller's next pushes.  begin takes the arguments of fddm_ddm_fft /
+ * fddm_ddm_diff (mode FDDM_MODE_FFT or FDDM_MODE_DIFF; nt ignored in diff
+ * mode); push takes `count` frames of height*width pixels in stack order;
+ * finish, once every frame is pushed, runs the remaining stages and writes
+ * the (num_lags+2, ny, nx/2+1) result to host memory -- the same values as
+ * fddm_ddm_fft / fddm_ddm_diff.  Device memory is held until destroy. */
+typedef struct fddm_ingest fddm_ingest;
+int fddm_ingest_begin(fddm_ingest **ingest, int mode, int dtype, uint64_t length, uint64_t height, uint64_t width,
+                      const uint32_t *lags, uint64_t num_lags, uint64_t nx, uint64_t ny, uint64_t nt,
+                      const void *window, int window_f32, int precision, int out_dtype, char *err, size_t errlen);
+int fddm_ingest_push(fddm_ingest *ingest, const void *frames, uint64_t count, char *err, size_t errlen);
+int fddm_ingest_finish(fddm_ingest *ingest, void *out, char *err, size_t errlen);
+int fddm_ingest_destroy(fddm_ingest *ingest);
+
 /* ---- plans: device-resident inputs and outputs ----
  * The phases of ddm_fft_cuda / ddm_diff_cuda (compute_fft2,
  * structure_function_fft/diff, make_shift; ddm_cuda.cu:40-1007) as one

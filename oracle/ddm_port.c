@@ -5,8 +5,9 @@
  * baseline.
  *
  * FP64 C restatement of the reference C++ core (src/cpp/ddm.cc), with an own
- * mixed-radix Stockham FFT in place of FFTW (FFTW's vendored source needs a
- * configure-generated config.h, so the reference core is not built here).
+ * mixed-radix Stockham FFT in place of FFTW (the vendored FFTW gets its
+ * config.h only from its own CMake build, lib/fftw-3.3.10/CMakeLists.txt:304,
+ * which is not run here, so the reference core is not built).
  *
  *   oracle_ddm_fft   <- ddm_fft<T>   src/cpp/ddm.cc:250-453
  *   oracle_ddm_diff  <- ddm_diff<T>  src/cpp/ddm.cc:37-192
