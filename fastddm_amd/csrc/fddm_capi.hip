@@ -577,9 +577,11 @@ struct HostPathCache {
   hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
+}  // namespace
+
 // memcpy split over host threads (pageable <-> pinned staging); the caller's
 // pages are touched by several threads at once.
-void parallel_memcpy(void* dst, const void* src, size_t n) {
+void fddm::parallel_memcpy(void* dst, const void* src, size_t n) {
   const size_t per = size_t(4) << 20;
   unsigned nt = static_cast<unsigned>(std::min<size_t>((n + per - 1) / per, 8));
   const unsigned hw = std::thread::hardware_concurrency();
@@ -598,6 +600,8 @@ void parallel_memcpy(void* dst, const void* src, size_t n) {
   std::memcpy(dst, src, std::min(n, slice));
   for (auto& t : th) t.join();
 }
+
+namespace {
 
 // Host -> device through the staging pair: host threads fill one buffer while
 // the DMA engine drains the other.

@@ -69,7 +69,8 @@ int set_err(char* err, size_t len, int code, const char* fmt, ...) {
     }                                                                                               \
   } while (0)
 
-constexpr uint64_t kChunkBytes = uint64_t(64) << 20;  // input bytes per chunk (target)
+constexpr uint64_t kChunkBytes = uint64_t(64) << 20;     // input bytes per chunk (target)
+constexpr uint64_t kMinStageBytes = uint64_t(4) << 20;   // smaller chunk buffers: result copied directly
 
 void free_all(fddm_ingest* h) {
   if (h->comp) (void)hipStreamSynchronize(h->comp);
@@ -186,7 +187,7 @@ int fddm_ingest_push(fddm_ingest* h, const void* frames, uint64_t count, char* e
   while (count) {
     if (h->staged == 0) FDDM_HIP(hipEventSynchronize(h->copied[h->cur]));  // its last DMA is done
     const uint64_t take = std::min(count, h->chunk_frames - h->staged);
-    std::memcpy(static_cast<unsigned char*>(h->pin[h->cur]) + h->staged * h->frame_b, src, take * h->frame_b);
+    parallel_memcpy(static_cast<unsigned char*>(h->pin[h->cur]) + h->staged * h->frame_b, src, take * h->frame_b);
     h->staged += take;
     h->pushed += take;
     src += take * h->frame_b;
@@ -211,8 +212,30 @@ int fddm_ingest_finish(fddm_ingest* h, void* out, char* err, size_t errlen) {
     h->failed = true;
     return rc;
   }
-  FDDM_HIP(hipMemcpyAsync(out, h->out, h->info.out_bytes, hipMemcpyDeviceToHost, h->comp));
-  FDDM_HIP(hipStreamSynchronize(h->comp));
+  // the result leaves through the two pinned chunk buffers (their frames are
+  // on the device by now: the compute stream waited for every copy): host
+  // threads drain one while the DMA engine fills the other
+  const uint64_t n = h->info.out_bytes, stage = h->chunk_frames * h->frame_b;
+  const auto* src = static_cast<const unsigned char*>(h->out);
+  auto* dst = static_cast<unsigned char*>(out);
+  if (stage < kMinStageBytes || n <= stage) {
+    FDDM_HIP(hipMemcpyAsync(out, h->out, n, hipMemcpyDeviceToHost, h->comp));
+    FDDM_HIP(hipStreamSynchronize(h->comp));
+    return FDDM_OK;
+  }
+  const uint64_t nchunk = (n + stage - 1) / stage;
+  for (uint64_t i = 0; i < nchunk + 2; ++i) {
+    if (i >= 2) {  // chunk i - 2 has arrived: copy it out, freeing its buffer
+      const uint64_t j = i - 2, off = j * stage, m = std::min(stage, n - off);
+      FDDM_HIP(hipEventSynchronize(h->copied[j & 1]));
+      parallel_memcpy(dst + off, h->pin[j & 1], m);
+    }
+    if (i < nchunk) {
+      const uint64_t off = i * stage, m = std::min(stage, n - off);
+      FDDM_HIP(hipMemcpyAsync(h->pin[i & 1], src + off, m, hipMemcpyDeviceToHost, h->comp));
+      FDDM_HIP(hipEventRecord(h->copied[i & 1], h->comp));
+    }
+  }
   return FDDM_OK;
 }
 

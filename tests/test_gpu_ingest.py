@@ -1,7 +1,8 @@
 """Streamed ingest (fddm_ingest_*): frames pushed in irregular pieces give the
 result of the one-call host path bit for bit -- fft and diff mode, fp32c and
 fp64, stacks whose length is not a multiple of the frame block, stacks of at
-most 16 frames (the FP64 edge child then takes every frame) -- and
+most 16 frames (the FP64 edge child then takes every frame), a result larger
+than the pinned chunk buffers it is copied out through -- and
 ddm_files on a multipage TIFF equals ddm on the array it holds."""
 
 import numpy as np
@@ -20,6 +21,8 @@ CASES = [
     ("fft", "fp64", (130, 48, 40), False, [64, 64]),
     ("diff", "fp32c", (90, 40, 40), True, [5]),
     ("fft", "fp32c", (12, 32, 32), False, [3, 4]),
+    # 26 MB chunk buffers, 53 MB result: the result leaves in three staged pieces
+    ("fft", "fp32c", (200, 256, 256), True, [64]),
 ]
 
 
