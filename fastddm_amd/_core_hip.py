@@ -59,6 +59,8 @@ def lib() -> ctypes.CDLL:
         "fddm_ddm_diff": [p, i32, u64, u64, u64, p, u64, u64, u64, p, i32, i32, i32, p, cp, sz],
         "fddm_release_cache": [cp, sz],
         "fddm_set_memory_budget": [u64, cp, sz],
+        "fddm_set_cache_limit": [ctypes.c_int64, cp, sz],
+        "fddm_get_cached_bytes": [p, cp, sz],
         "fddm_get_last_call_chunks": [ctypes.POINTER(u64), cp, sz],
         "fddm_ingest_begin": [ctypes.POINTER(p), i32, i32, u64, u64, u64, p, u64, u64, u64, u64, p, i32, i32, i32,
                               cp, sz],
@@ -124,6 +126,8 @@ EXPORTED_SYMBOLS = (
     "fddm_ddm_diff",
     "fddm_release_cache",
     "fddm_set_memory_budget",
+    "fddm_set_cache_limit",
+    "fddm_get_cached_bytes",
     "fddm_get_last_call_chunks",
     "fddm_ingest_begin",
     "fddm_ingest_push",
@@ -244,6 +248,20 @@ def check_host_memory_diff(nx, ny, length, num_lags, out_dtype=np.float64) -> bo
 
 def release_cache() -> None:
     _call("fddm_release_cache")
+
+
+def set_cache_limit(nbytes: int) -> None:
+    """Device bytes of buffers ``ddm_fft`` / ``ddm_diff`` keep on the current
+    device for the next call (negative: the default, 16 GiB; 0: none).  A
+    call needing more frees its buffers before returning."""
+    _call("fddm_set_cache_limit", int(nbytes))
+
+
+def cached_bytes() -> int:
+    """Device bytes the host-path calls keep cached on the current device."""
+    v = ctypes.c_uint64(0)
+    _call("fddm_get_cached_bytes", ctypes.byref(v))
+    return v.value
 
 
 def set_memory_budget(nbytes: int) -> None:

@@ -642,6 +642,23 @@ int d2h_staged(HostPathCache& c, void* h_dst, const void* d_src, size_t n, char*
 }
 std::mutex g_cache_mu[64];  // one per device: calls on different devices do not wait for each other
 HostPathCache g_cache[64];
+// Device bytes of input, output and workspace buffers kept for the next
+// host-buffer call on a device (kCacheKeepBytes until set).  Larger sets are
+// freed when the call returns.
+constexpr uint64_t kCacheKeepBytes = uint64_t(16) << 30;
+bool g_cache_limit_set[64] = {};
+uint64_t g_cache_limit[64] = {};
+
+uint64_t cache_limit(int dev) { return g_cache_limit_set[dev] ? g_cache_limit[dev] : kCacheKeepBytes; }
+
+// Frees the cached input, output and workspace buffers (staging buffers and
+// the stream stay); the caller holds the device's cache lock.
+void drop_buffers(HostPathCache& c) {
+  for (void** ptr : {&c.img, &c.out, &c.ws})
+    if (*ptr) (void)hipFree(*ptr);
+  c.img = c.out = c.ws = nullptr;
+  c.img_cap = c.out_cap = c.ws_cap = 0;
+}
 
 int grow(void** ptr, uint64_t* cap, uint64_t need, char* err, size_t errlen) {
   if (*cap >= need && *ptr) return FDDM_OK;
@@ -1048,10 +1065,7 @@ int host_path(int mode, const void* img, int dtype, uint64_t L, uint64_t H, uint
   if (img_bytes + out_b + p->ws_bytes > budget) {
     // in pieces: drop the cached buffers first so the pieces fit beside the rest
     FDDM_HIP(hipStreamSynchronize(c.stream));
-    for (void** ptr : {&c.img, &c.out, &c.ws})
-      if (*ptr) (void)hipFree(*ptr);
-    c.img = c.out = c.ws = nullptr;
-    c.img_cap = c.out_cap = c.ws_cap = 0;
+    drop_buffers(c);
     FDDM_HIP(hipMemGetInfo(&free_b, &total_b));
     const uint64_t avail = std::min<uint64_t>(budget, static_cast<uint64_t>(0.95 * static_cast<double>(free_b)));
     ChunkPlan cp;
@@ -1078,6 +1092,7 @@ int host_path(int mode, const void* img, int dtype, uint64_t L, uint64_t H, uint
   if ((rc = plan_execute(p, c.img, c.out, c.ws, c.stream, err, errlen))) return rc;
   if ((rc = d2h_staged(c, out, c.out, out_b, err, errlen))) return rc;
   FDDM_HIP(hipStreamSynchronize(c.stream));
+  if (c.img_cap + c.out_cap + c.ws_cap > cache_limit(dev)) drop_buffers(c);
   return FDDM_OK;
 }
 
@@ -1209,8 +1224,7 @@ int fddm_release_cache(char* err, size_t errlen) {
   std::lock_guard<std::mutex> lk(g_cache_mu[dev]);
   HostPathCache& c = g_cache[dev];
   if (c.stream) FDDM_HIP(hipStreamSynchronize(c.stream));
-  for (void* ptr : {c.img, c.out, c.ws})
-    if (ptr) (void)hipFree(ptr);
+  drop_buffers(c);
   for (int k = 0; k < 2; ++k) {
     if (c.pin[k]) (void)hipHostFree(c.pin[k]);
     if (c.ev[k]) (void)hipEventDestroy(c.ev[k]);
@@ -1218,6 +1232,32 @@ int fddm_release_cache(char* err, size_t errlen) {
   if (c.stream) (void)hipStreamDestroy(c.stream);
   c = HostPathCache{};
   azavg_release_scratch(dev);
+  return FDDM_OK;
+}
+
+int fddm_get_cached_bytes(uint64_t* bytes, char* err, size_t errlen) {
+  if (!bytes) return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument");
+  int dev = 0;
+  FDDM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(err, errlen, FDDM_ERR_DEVICE, "device id %d not supported", dev);
+  std::lock_guard<std::mutex> lk(g_cache_mu[dev]);
+  const HostPathCache& c = g_cache[dev];
+  *bytes = c.img_cap + c.out_cap + c.ws_cap;
+  return FDDM_OK;
+}
+
+int fddm_set_cache_limit(int64_t bytes, char* err, size_t errlen) {
+  int dev = 0;
+  FDDM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(err, errlen, FDDM_ERR_DEVICE, "device id %d not supported", dev);
+  std::lock_guard<std::mutex> lk(g_cache_mu[dev]);
+  g_cache_limit_set[dev] = bytes >= 0;
+  g_cache_limit[dev] = bytes >= 0 ? static_cast<uint64_t>(bytes) : 0;
+  HostPathCache& c = g_cache[dev];
+  if (c.img_cap + c.out_cap + c.ws_cap > cache_limit(dev)) {
+    if (c.stream) FDDM_HIP(hipStreamSynchronize(c.stream));
+    drop_buffers(c);
+  }
   return FDDM_OK;
 }
 

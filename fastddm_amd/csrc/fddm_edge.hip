@@ -16,9 +16,9 @@
 // 1e-7 of sum_t |G|^2 in (L - dt) D.  Where (L - dt) D is small against that
 // sum -- slow low-q modes and chance near-recurrences -- the kernel marks the
 // q, and k_wk_fix recomputes the affected lags of the marked q by direct
-// differences of the stored float series in double: workgroups stride over
-// the columns, each marked one's lags are shared by the workgroup's waves,
-// lanes run over t and a wave sum closes each lag.
+// differences of the stored float series in double: each workgroup lists its
+// marked columns (flags read in parallel), each marked one's lags are shared
+// by the workgroup's waves, lanes run over t and a wave sum closes each lag.
 #include <algorithm>
 
 #include "fddm_kernels.hpp"
@@ -51,6 +51,11 @@ __global__ void __launch_bounds__(kThreads)
 // (those with n = L - lags[i] <= E), D = sum_{t < n} |F(t + dt) - F(t)|^2 / n
 // from the FP64 edge spectra S64[q][c] (row pitch lp), where frame f sits in
 // column f if f < E or the stack is short (L <= 2E), else f - (L - 2E).
+// For E = kEdgeFrames and L > 2E the 2E values are loaded once (16 loads in
+// flight) and D depends on n alone: frame t + dt = L - n + t sits in column
+// 2E - n + t, so the E sums run over registers with compile-time indices
+// and are then stored to the lag rows they belong to (same order of
+// additions as the general loop, so the same values).
 template <typename OutT>
 __global__ void __launch_bounds__(kThreads)
     k_edge_fix(TimeArgs a, const double2* __restrict__ S64, uint64_t lp, int E, int i0) {
@@ -61,6 +66,33 @@ __global__ void __launch_bounds__(kThreads)
   OutT* __restrict__ out = static_cast<OutT*>(a.out);
   const size_t o = col_out(a, c), ldo = a.ldo;
   const double2* __restrict__ f = S64 + static_cast<size_t>(q) * lp;
+  if (E == kEdgeFrames && L > 2 * kEdgeFrames) {
+    constexpr int EE = kEdgeFrames;
+    double2 v[2 * EE];
+#pragma unroll
+    for (int k = 0; k < 2 * EE; ++k) v[k] = f[k];
+    double dn[EE];  // dn[n - 1]: D of the lag with n pairs
+#pragma unroll
+    for (int n = 1; n <= EE; ++n) {
+      double acc = 0.0;
+#pragma unroll
+      for (int t = 0; t < n; ++t) {
+        const double2 x = v[2 * EE - n + t], y = v[t];
+        const double dr = x.x - y.x, di = x.y - y.y;
+        acc += dr * dr + di * di;
+      }
+      dn[n - 1] = acc / n;
+    }
+    for (int i = i0; i < nl; ++i) {
+      const int n = L - static_cast<int>(lags[i]);  // 1 <= n <= EE
+      double D = dn[0];
+#pragma unroll
+      for (int k = 1; k < EE; ++k)
+        if (n == k + 1) D = dn[k];
+      out[static_cast<size_t>(i) * ldo + o] = static_cast<OutT>(D);
+    }
+    return;
+  }
   const int shift = L <= 2 * E ? 0 : L - 2 * E;
   for (int i = i0; i < nl; ++i) {
     const int d = static_cast<int>(lags[i]), n = L - d;
@@ -78,43 +110,59 @@ __global__ void __launch_bounds__(kThreads)
 
 template <typename OutT>
 __global__ void __launch_bounds__(kThreads) k_wk_fix(TimeArgs a) {
-  // workgroups stride over the launch's columns; the waves of a workgroup
-  // share the lags of each marked column
+  // workgroup g owns the columns g + m G (G workgroups); their flags are read
+  // at once, one per thread, the marked columns listed in LDS, and the
+  // waves of the workgroup share the lags of each listed column
+  __shared__ int list[kThreads];
+  __shared__ int count;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int L = a.L, nl = a.nl;
+  const int L = a.L, nl = a.nl, G = static_cast<int>(gridDim.x), g = static_cast<int>(blockIdx.x);
   const uint32_t* __restrict__ lags = a.lags;
   OutT* __restrict__ out = static_cast<OutT*>(a.out);
   const size_t ldo = a.ldo;
-  for (int c = blockIdx.x; c < a.cols; c += gridDim.x) {
-    const int q = col_q(a, c);
-    if (a.flags[q] == 0) continue;  // uniform over the workgroup
-    const size_t oq = col_out(a, c);
-    const double var = static_cast<double>(out[static_cast<size_t>(nl + 1) * ldo + oq]);
-    const double thr = kWkFlag * static_cast<double>(L) * var;
-    const float2* __restrict__ f = static_cast<const float2*>(a.S) + static_cast<size_t>(q - a.s_q0) * a.Lp;
-    for (int b = wave * 64; b < nl; b += kThreads) {
-      const int i = b + lane;
-      bool need = false;
-      if (i < nl) {
-        const int n = L - static_cast<int>(lags[i]);
-        need = n > 32 && static_cast<double>(n) * static_cast<double>(out[static_cast<size_t>(i) * ldo + oq]) < thr;
-      }
-      uint64_t todo = __ballot(need);
-      while (todo) {
-        const int ii = b + __ffsll(static_cast<long long>(todo)) - 1;
-        todo &= todo - 1;
-        const int d = static_cast<int>(lags[ii]), n = L - d;
-        double acc = 0.0;
-        for (int t = lane; t < n; t += 64) {
-          const float2 x = f[t + d], y = f[t];
-          const double dr = static_cast<double>(x.x) - static_cast<double>(y.x);
-          const double di = static_cast<double>(x.y) - static_cast<double>(y.y);
-          acc += dr * dr + di * di;
+  const int M = (a.cols - g + G - 1) / G;  // columns of this workgroup (g < cols)
+  for (int m0 = 0; m0 < M; m0 += kThreads) {
+    if (threadIdx.x == 0) count = 0;
+    __syncthreads();
+    const int m = m0 + static_cast<int>(threadIdx.x);
+    if (m < M) {
+      const int c = g + m * G;
+      if (a.flags[col_q(a, c)] != 0) list[atomicAdd(&count, 1)] = c;
+    }
+    __syncthreads();
+    const int nmark = count;
+    for (int k = 0; k < nmark; ++k) {
+      const int c = list[k];
+      const int q = col_q(a, c);
+      const size_t oq = col_out(a, c);
+      const double var = static_cast<double>(out[static_cast<size_t>(nl + 1) * ldo + oq]);
+      const double thr = kWkFlag * static_cast<double>(L) * var;
+      const float2* __restrict__ f = static_cast<const float2*>(a.S) + static_cast<size_t>(q - a.s_q0) * a.Lp;
+      for (int b = wave * 64; b < nl; b += kThreads) {
+        const int i = b + lane;
+        bool need = false;
+        if (i < nl) {
+          const int n = L - static_cast<int>(lags[i]);
+          need = n > 32 && static_cast<double>(n) * static_cast<double>(out[static_cast<size_t>(i) * ldo + oq]) < thr;
         }
-        acc = wave::incl_scan(acc);
-        if (lane == 63) out[static_cast<size_t>(ii) * ldo + oq] = static_cast<OutT>(acc / n);
+        uint64_t todo = __ballot(need);
+        while (todo) {
+          const int ii = b + __ffsll(static_cast<long long>(todo)) - 1;
+          todo &= todo - 1;
+          const int d = static_cast<int>(lags[ii]), n = L - d;
+          double acc = 0.0;
+          for (int t = lane; t < n; t += 64) {
+            const float2 x = f[t + d], y = f[t];
+            const double dr = static_cast<double>(x.x) - static_cast<double>(y.x);
+            const double di = static_cast<double>(x.y) - static_cast<double>(y.y);
+            acc += dr * dr + di * di;
+          }
+          acc = wave::incl_scan(acc);
+          if (lane == 63) out[static_cast<size_t>(ii) * ldo + oq] = static_cast<OutT>(acc / n);
+        }
       }
     }
+    __syncthreads();  // list and count are rewritten by the next round
   }
 }
 

@@ -385,16 +385,19 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
         constexpr int STEP = Lay::NT / G;
         constexpr int KL = (NH + STEP - 1) / STEP;
         constexpr bool PRE = Lay::kPair || SEQ > 1 || F64;
+        // (cos, sin)(2 pi d / N) of lag d: unsigned 32-bit indices, so each
+        // entry address is one 64-bit shift-add from this base
+        const C* __restrict__ const twp = tw + TWP;
         int dk[PRE ? KL : 1];
         C csk[PRE ? KL : 1];
         if constexpr (PRE) {
 #pragma unroll
           for (int k = 0; k < KL; ++k) {
             const int i = a.nl - 1 - v - k * STEP;
-            dk[k] = i >= 0 ? static_cast<int>(a.lags[i]) : 1;
+            dk[k] = i >= 0 ? static_cast<int>(a.lags[static_cast<uint32_t>(i)]) : 1;
           }
 #pragma unroll
-          for (int k = 0; k < KL; ++k) csk[k] = tw[TWP + dk[k]];
+          for (int k = 0; k < KL; ++k) csk[k] = twp[static_cast<uint32_t>(dk[k])];
         }
         auto lag_value = [&](int d, C cs) -> Acc {
           Acc D;
@@ -460,7 +463,7 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
           } else {
             for (int i = a.nl - 1 - v; i >= 0; i -= STEP) {
               const int d = static_cast<int>(a.lags[i]);
-              out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(lag_value(d, tw[TWP + d]));
+              out[static_cast<size_t>(i) * nqs + oq] = static_cast<OutT>(lag_value(d, twp[static_cast<uint32_t>(d)]));
             }
           }
         }

@@ -104,6 +104,19 @@ int fddm_ddm_diff(const void *img, int dtype, uint64_t length, uint64_t height, 
 /* Drops the device buffers cached by fddm_ddm_fft / fddm_ddm_diff and the
  * scratch kept by fddm_azimuthal_average, on the current device. */
 int fddm_release_cache(char *err, size_t errlen);
+/* Device bytes of input, output and workspace buffers that fddm_ddm_fft /
+ * fddm_ddm_diff keep on the current device for the next call (a negative
+ * value restores the default, 16 GiB; 0 keeps none).  A call whose buffers
+ * exceed it frees them before returning, as the reference frees its device
+ * buffers at the end of every call (ddm_cuda.cu:250-260, :524-529,
+ * :917-922); a lower
+ * limit frees what is cached now.  No counterpart in the reference (which
+ * keeps nothing between calls). */
+int fddm_set_cache_limit(int64_t bytes, char *err, size_t errlen);
+/* Device bytes of input, output and workspace buffers cached on the current
+ * device now (the runtime may keep freed device memory in its own pools, so
+ * free-memory queries need not show a release at once). */
+int fddm_get_cached_bytes(uint64_t *bytes, char *err, size_t errlen);
 /* Device memory fddm_ddm_fft / fddm_ddm_diff may use on the current device
  * (0: 95 % of free memory, the default).  A call whose input, workspace and
  * output exceed it runs in pieces -- the spatial stage over chunks of frames,

@@ -70,3 +70,30 @@ def test_budget_chunks_equal_one_shot(hip, mode, precision, shape, window, lagsp
             run()
     finally:
         hip.set_memory_budget(0)
+
+
+def test_cache_limit(hip):
+    """Buffers above the cache limit are freed when the call returns; within
+    it they stay for the next call (same result either way)."""
+    L, H, W = 400, 256, 256
+    img = ar1_speckle(L, H, W, seed=7)
+    lags = np.arange(1, L)
+    nt = next_fast_len(2 * L, "hip")
+    try:
+        hip.release_cache()
+        assert hip.cached_bytes() == 0
+        hip.set_cache_limit(0)
+        a = hip.ddm_fft(img, lags, W, H, nt, None, precision="fp32c")
+        assert hip.cached_bytes() == 0
+        hip.set_cache_limit(-1)
+        b = hip.ddm_fft(img, lags, W, H, nt, None, precision="fp32c")
+        # input 52 MB, spectra 2 x 106 MB, output 106 MB stay on the device
+        assert hip.cached_bytes() >= 300 << 20
+        assert np.array_equal(a, b)
+        hip.set_cache_limit(100 << 20)  # a lower limit frees what is cached now
+        assert hip.cached_bytes() == 0
+        c = hip.ddm_fft(img, lags, W, H, nt, None, precision="fp32c")
+        assert hip.cached_bytes() == 0 and np.array_equal(a, c)
+    finally:
+        hip.set_cache_limit(-1)
+        hip.release_cache()
