@@ -585,6 +585,7 @@ def main():
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-large", action="store_true", help="skip the 1024^2 x 4000 leg")
     ap.add_argument("--skip-diff-leg", action="store_true", help="skip the diff-mode leg")
+    ap.add_argument("--skip-host", action="store_true", help="skip the host-to-host and streamed-ingest legs")
     ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving measurement")
     ap.add_argument("--precision-report", action="store_true",
@@ -684,34 +685,35 @@ def main():
     log(f"fp32c: {ms32:.3f} ms/step, stages {st32}")
     # host numpy in -> host numpy out through the C ABI (PCIe both ways, device
     # buffers cached after the first call); reported beside, never as value
-    e2e = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        if diff:
-            core.ddm_diff(img, lags, nx, ny, win, precision="fp32c")
-        else:
-            core.ddm_fft(img, lags, nx, ny, nt, win, precision="fp32c")
-        e2e.append(time.perf_counter() - t0)
-    e2e_s = min(e2e[1:])
-    core.release_cache()
-    log(f"host-to-host fp32c: {e2e_s * 1e3:.1f} ms per call")
-    # the same analysis from host frames pushed in batches of 64 (streamed
-    # ingest: pinned chunks, DMA and row/column passes overlapped)
-    ingest_s, ingest_parts = [], []
-    for _ in range(2):
-        t0 = time.perf_counter()
-        ing = core.Ingest(mode, img.dtype, img.shape, lags, nx, ny, 0 if diff else nt, win, "fp32c")
-        t1 = time.perf_counter()
-        for k in range(0, L, 64):
-            ing.push(img[k:k + 64])
-        t2 = time.perf_counter()
-        ing.finish()
-        t3 = time.perf_counter()
-        ing.close()
-        ingest_s.append(time.perf_counter() - t0)
-        ingest_parts.append({"begin_ms": (t1 - t0) * 1e3, "push_ms": (t2 - t1) * 1e3, "finish_ms": (t3 - t2) * 1e3,
-                             "close_ms": (ingest_s[-1] - (t3 - t0)) * 1e3})
-    log(f"streamed ingest fp32c: {min(ingest_s) * 1e3:.1f} ms per analysis")
+    e2e_s, ingest_s, ingest_parts = None, [], []
+    if not args.skip_host:
+        e2e = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            if diff:
+                core.ddm_diff(img, lags, nx, ny, win, precision="fp32c")
+            else:
+                core.ddm_fft(img, lags, nx, ny, nt, win, precision="fp32c")
+            e2e.append(time.perf_counter() - t0)
+        e2e_s = min(e2e[1:])
+        core.release_cache()
+        log(f"host-to-host fp32c: {e2e_s * 1e3:.1f} ms per call")
+        # the same analysis from host frames pushed in batches of 64 (streamed
+        # ingest: pinned chunks, DMA and row/column passes overlapped)
+        for _ in range(2):
+            t0 = time.perf_counter()
+            ing = core.Ingest(mode, img.dtype, img.shape, lags, nx, ny, 0 if diff else nt, win, "fp32c")
+            t1 = time.perf_counter()
+            for k in range(0, L, 64):
+                ing.push(img[k:k + 64])
+            t2 = time.perf_counter()
+            ing.finish()
+            t3 = time.perf_counter()
+            ing.close()
+            ingest_s.append(time.perf_counter() - t0)
+            ingest_parts.append({"begin_ms": (t1 - t0) * 1e3, "push_ms": (t2 - t1) * 1e3, "finish_ms": (t3 - t2) * 1e3,
+                                 "close_ms": (ingest_s[-1] - (t3 - t0)) * 1e3})
+        log(f"streamed ingest fp32c: {min(ingest_s) * 1e3:.1f} ms per analysis")
     fp64 = None
     b64 = algorithmic_bytes(L, S, S, img.itemsize, nq, len(lags), 16, 8, win is not None)
     if not args.skip_fp64:
@@ -778,14 +780,14 @@ def main():
             "traffic_source": traffic_src,
             "algorithmic_bytes": dom_bytes,
         },
-        "host_to_host": {
+        "host_to_host": None if e2e_s is None else {
             "frames_per_s": L / e2e_s,
             "ms_per_call": e2e_s * 1e3,
             "note": (f"{'ddm_diff' if diff else 'ddm_fft'} through the C ABI from a host numpy stack to a "
                      f"host numpy result (H2D of the {img.dtype} frames and D2H of the float64 output "
                      "included)"),
         },
-        "ingest": {
+        "ingest": None if not ingest_s else {
             "frames_per_s": L / min(ingest_s),
             "ms_per_analysis": min(ingest_s) * 1e3,
             "phases_ms": {k: round(v, 2) for k, v in ingest_parts[int(np.argmin(ingest_s))].items()},

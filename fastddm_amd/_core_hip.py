@@ -401,6 +401,8 @@ class DeviceBuffer:
 
     def copy_from(self, other: "DeviceBuffer", nbytes: int) -> None:
         """Device-to-device copy of nbytes from `other` into this buffer."""
+        if nbytes < 0 or nbytes > self.nbytes or nbytes > other.nbytes:
+            raise RuntimeError(f"copy of {nbytes} bytes between buffers of {other.nbytes} and {self.nbytes} bytes")
         _call("fddm_memcpy_dtod", self.ptr, other.ptr, int(nbytes))
 
     def free(self) -> None:
