@@ -142,36 +142,49 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems
 // of k_rows_p2 with one wave per pair row.  Workgroup b takes items b,
 // b + grid, ... (item = tb * H + y); wave p loads, converts and transforms
 // pair row p in its own LDS region with no workgroup barrier, loading the
-// next item's pixels and frame-0 row into registers before the transform so
-// their latency overlaps it.  Two barriers per item remain: before the split
-// into half spectra (which reads every pair row) and before the next item
-// rewrites the rows.
+// next item's pixels, frame-0 row and window values into registers before the
+// transform so their latency overlaps it.  Two barriers per item remain:
+// before the split into half spectra (which reads every pair row) and before
+// the next item rewrites the rows.  The twiddles of passes 1 and 2 are copied
+// into LDS once per workgroup: read from global memory inside the passes,
+// their waits (vmcnt, which also counts the prefetch loads and the previous
+// item's stores) would hold each pass until those had completed.
 template <typename Tin, typename T, int LOGX, int NP>
 __global__ void __launch_bounds__(64 * NP, 8) k_rows_p2_pf(RowArgs a, uint32_t nitems) {
   using C = cplx_t<T>;
   constexpr int NT = 64 * NP;
   constexpr int NX = 1 << LOGX, NXH = NX / 2 + 1, TB = 2 * NP;
   constexpr SpPlan P = sp_plan(LOGX);
+  constexpr int TWN = P.r0 * P.r1 * (1 + P.r2);  // [k][r] tables of passes 1 and 2
   constexpr int VX = 8 / sizeof(Tin), GV = NX / VX;
   constexpr int PER = GV / 64;  // 8-byte vectors per lane and frame row
   static_assert(GV % 64 == 0, "whole vectors per lane");
   struct alignas(8) Vec {
     Tin v[VX];
   };
+  struct alignas(VX * sizeof(T)) WVec {
+    T v[VX];
+  };
   extern __shared__ __align__(16) unsigned char smem[];
   C* const buf = reinterpret_cast<C*>(smem);
-  const int tid = threadIdx.x, lane = tid & 63, p = tid >> 6;  // p: this wave's pair row
+  C* const twl = buf + lds_padded_size(NP * NX);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int p = __builtin_amdgcn_readfirstlane(tid >> 6);  // this wave's pair row (uniform)
   C* const row = buf + padi(p * NX);  // p NX is a multiple of 32: padded offsets add
   const int H = a.H;
   const Tin* __restrict__ img = static_cast<const Tin*>(a.img);
   const T* __restrict__ win = static_cast<const T*>(a.window);
   const size_t plane = static_cast<size_t>(H) * NX;
   const bool sub = a.subtract0 != 0;
-  const C* __restrict__ tw = static_cast<const C*>(a.twx);
   const T half_norm = static_cast<T>(0.5 * a.norm);
   const uint32_t kstride = static_cast<uint32_t>(H) * TB;
+  {
+    const C* __restrict__ tw = static_cast<const C*>(a.twx);
+    for (int i = tid; i < TWN; i += NT) twl[i] = tw[i];
+  }
 
   Vec va[PER], vb[PER], v0[PER];
+  WVec wv[PER];
   auto load = [&](uint32_t item) {
     const int y = static_cast<int>(item % H);
     const uint64_t ta = static_cast<uint64_t>(item / H) * TB + 2 * p;
@@ -183,14 +196,17 @@ __global__ void __launch_bounds__(64 * NP, 8) k_rows_p2_pf(RowArgs a, uint32_t n
       va[k] = ta < a.frames ? *reinterpret_cast<const Vec*>(row0 + ta * plane + x0) : Vec{};
       vb[k] = ta + 1 < a.frames ? *reinterpret_cast<const Vec*>(row0 + (ta + 1) * plane + x0) : Vec{};
       v0[k] = sub ? *reinterpret_cast<const Vec*>(ref0 + x0) : Vec{};
+      if (win) wv[k] = *reinterpret_cast<const WVec*>(win + static_cast<size_t>(y) * NX + x0);
     }
   };
 
   uint32_t item = blockIdx.x;
   if (item < nitems) load(item);
+  __syncthreads();  // twiddles in LDS
   for (; item < nitems; item += gridDim.x) {
     const int y = static_cast<int>(item % H);
     const uint64_t tb = item / H, ta = tb * TB + 2 * p;
+    const bool has_a = ta < a.frames, has_b = ta + 1 < a.frames;  // uniform over the wave
     // registers -> this wave's row: frames (2p, 2p+1) as one complex row
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
@@ -198,15 +214,15 @@ __global__ void __launch_bounds__(64 * NP, 8) k_rows_p2_pf(RowArgs a, uint32_t n
       C* d = row + padi(x0);  // VX divides 32: the VX slots are contiguous
 #pragma unroll
       for (int v = 0; v < VX; ++v) {
-        const T w = win ? win[static_cast<size_t>(y) * NX + x0 + v] : T(1);
-        const T fa = ta < a.frames ? pixel_value<Tin, T>(va[k].v[v], v0[k].v[v], sub, w) : T(0);
-        const T fb = ta + 1 < a.frames ? pixel_value<Tin, T>(vb[k].v[v], v0[k].v[v], sub, w) : T(0);
+        const T w = win ? wv[k].v[v] : T(1);
+        const T fa = has_a ? pixel_value<Tin, T>(va[k].v[v], v0[k].v[v], sub, w) : T(0);
+        const T fb = has_b ? pixel_value<Tin, T>(vb[k].v[v], v0[k].v[v], sub, w) : T(0);
         d[v] = C{fa, fb};
       }
     }
     wave_lds_sync();
     if (item + gridDim.x < nitems) load(item + gridDim.x);
-    fft3_wave<T, NX, P.r0, P.r1, P.r2>(row, tw, lane);
+    fft3_wave<T, NX, P.r0, P.r1, P.r2>(row, twl, lane);
     __syncthreads();  // every pair row transformed
 
     // split into the two half spectra and store R[tb][k][y][ti] (ti fastest)
@@ -306,12 +322,15 @@ __global__ void __launch_bounds__(NT) k_cols_p2(ColArgs a, uint32_t nitems) {
 
 template <int LOGX, typename Tin, typename T, int NP>
 hipError_t rows_np(const RowArgs& a, unsigned grid, hipStream_t s) {
-  const size_t lds = sizeof(cplx_t<T>) * lds_padded_size(NP * (1 << LOGX));
+  const size_t lds_rows = sizeof(cplx_t<T>) * lds_padded_size(NP * (1 << LOGX));
+  const size_t lds = lds_rows;
   // persistent form (FP32, 2-byte pixels, rows up to 512 wide: the cases whose
   // prefetch registers fit 8 waves per SIMD without spills); as many
   // workgroups as stay resident
   if constexpr (sizeof(Tin) == 2 && sizeof(T) == 4 && LOGX <= 9) {
     if (a.W == (1 << LOGX)) {
+      constexpr SpPlan P = sp_plan(LOGX);
+      const size_t lds = lds_rows + sizeof(cplx_t<T>) * P.r0 * P.r1 * (1 + P.r2);  // + the LDS twiddles
       auto kernel = k_rows_p2_pf<Tin, T, LOGX, NP>;
       if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),

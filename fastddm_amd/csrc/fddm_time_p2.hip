@@ -15,10 +15,13 @@
 //   2. compensated paths: subtracts the temporal mean (one group reduction);
 //   3. FFT1 (radices R0 .. RL); the last pass stays in registers, |G|^2 is
 //      formed there and written to LDS as z(n) = P(2n) + i P(2n+1);
-//   4. re-reads the series (cache resident) for the prefix sums of |F(t)|^2
-//      (double on the compensated paths; contiguous segments per thread and
-//      one group scan) into one half of the buffer, and keeps the first and
-//      last kEdge samples;
+//   4. forms the prefix sums of |F(t)|^2 (double on the compensated paths)
+//      and keeps the first and last kEdge samples: from the first pass's
+//      registers before step 3 (wave scans per segment of the register
+//      layout, held in registers until a buffer half is free), or, on the
+//      paired N = 4096 plans and at N = 16384, whose registers are full, from
+//      a second (cache-resident) read of the series with contiguous segments
+//      per thread and one group scan;
 //   5. the second forward transform of the real P runs as one complex
 //      transform of length N/2 of z (p2_half_plan) in the other half (in
 //      place, or alternating between the halves before step 4 on the paired
@@ -111,11 +114,20 @@ struct P2Lds {
   static constexpr int N = 1 << LOGN, TPT = P.threads, WPT = TPT / 64;
   using Acc = std::conditional_t<SUB, double, float>;
   static constexpr size_t csz = sizeof(cplx_t<T>);
-  static constexpr size_t group_bytes =
-      lds_padded_size(N) * csz + 2 * kEdge * csz + 2 * WPT * sizeof(double) + WPT * sizeof(Acc);
+  static constexpr int S = N / 2 / TPT;  // first-pass segments of TPT samples below N/2
+  static constexpr size_t group_bytes0 =
+      lds_padded_size(N) * csz + 2 * kEdge * csz + 2 * WPT * sizeof(double) + (S * WPT + 1) * sizeof(Acc);
   // (the uncompensated FP32 N = 4096 kernel spills 12 VGPRs at 8 waves per
   // SIMD and is still faster paired: 3.28 -> 3.09 ms at 512^2 x 2000)
-  static constexpr bool kPair = TPT >= 256 && TPT <= 512 && 4 * group_bytes <= 160 * 1024;
+  static constexpr bool kPair = TPT >= 256 && TPT <= 512 && 4 * group_bytes0 <= 160 * 1024;
+  // Prefix sums formed from the first pass's registers (no second read of the
+  // series) where the registers allow it: not on the paired plans, whose 8
+  // waves per SIMD leave no room to hold S sums across the first transform,
+  // nor at N = 16384.  Those re-read the series from cache.
+  static constexpr bool REG = !kPair && LOGN <= 13;
+  // per q group: wave totals (REG: by segment and wave, then their offsets)
+  // and, last, their sum
+  static constexpr int CH = (REG ? S : 1) * WPT + 1;
   static constexpr int NT = TPT < kMinThreads ? kMinThreads : (kPair ? 2 * TPT : TPT);  // workgroup size
   static constexpr int G = NT / TPT;
   // Plans from N = 8192 (FP64 from N = 4096) hold one q per workgroup (the
@@ -134,7 +146,7 @@ struct P2Lds {
   static constexpr size_t buf = static_cast<size_t>(G) * lds_padded_size(N) * csz;
   static constexpr size_t edge = static_cast<size_t>(G) * 2 * kEdge * csz;
   static constexpr size_t red = 2 * (NT / 64) * sizeof(double);
-  static constexpr size_t chunk = static_cast<size_t>(G) * WPT * sizeof(Acc);
+  static constexpr size_t chunk = static_cast<size_t>(G) * CH * sizeof(Acc);
   static constexpr size_t bytes = buf + edge + red + chunk;
 };
 
@@ -226,7 +238,7 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
     Acc* const chunk0 = reinterpret_cast<Acc*>(smem + Lay::buf + Lay::edge + Lay::red);
     C* const buf = buf0 + g * lds_padded_size(N);
     C* const edge = edge0 + g * 2 * kEdge;
-    Acc* const chunk = chunk0 + g * WPT;
+    Acc* const chunk = chunk0 + g * Lay::CH;
     const int c = SEQ > 1 ? SEQ * wgi + sq : cbase + g;  // column of the launch
     const bool active = c < nq;
     const int q = active ? col_q(a, c) : a.s_q0;
@@ -281,6 +293,51 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
     const double fsr = sr - static_cast<double>(L) * static_cast<double>(m_r);
     const double fsi = si - static_cast<double>(L) * static_cast<double>(m_i);
 
+    // 4. (REG) prefix sums psum(t + 1) of |F(t) - mean|^2, from the first
+    // pass's registers: they hold t = u + b TPT + r M0 = s TPT + u (segment
+    // s = b + r MB0), so each segment is one wave scan per wave, then the
+    // wave totals are offset in t order by one wave of the group, and the
+    // sums stay in registers (ps[s]) until a buffer half is free for them.
+    // The first and last kEdge samples go to `edge` for the direct sums.
+    constexpr int S = Lay::S;
+    static_assert(S == RH * MB0, "first-pass registers cover t < N/2");
+    Acc ps[Lay::REG ? S : 1];
+    if constexpr (Lay::REG) {
+#pragma unroll
+      for (int b = 0; b < MB0; ++b)
+#pragma unroll
+        for (int r = 0; r < RH; ++r) {
+          const int sg = b + r * MB0, t = sg * TPT + u;
+          const C f = v0[b][r];  // zero for t >= L and for an inactive column
+          if constexpr (SUB) {
+            if (t < kEdge && t < L) edge[t] = f;
+            if (t >= L - kEdge && t < L) edge[t - L + 2 * kEdge] = f;
+          }
+          ps[sg] = wave::incl_scan(static_cast<Acc>(f.x) * static_cast<Acc>(f.x) +
+                                   static_cast<Acc>(f.y) * static_cast<Acc>(f.y));
+          if (lane == 63) chunk[sg * WPT + wg] = ps[sg];
+        }
+      __syncthreads();
+      if (wg == 0) {
+        // exclusive offsets of the (segment, wave) totals in t order, and their sum
+        constexpr int E = S * WPT;
+        Acc carry = 0;
+#pragma unroll
+        for (int e0 = 0; e0 < E; e0 += 64) {
+          const int e = e0 + lane;
+          const Acc x = e < E ? chunk[e] : Acc(0);
+          const Acc inc = wave::incl_scan(x);
+          const Acc exc = carry + wave::from_prev_lane(inc);
+          carry += wave::lane63(inc);
+          if (e < E) chunk[e] = exc;
+        }
+        if (lane == 0) chunk[E] = carry;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int sg = 0; sg < S; ++sg) ps[sg] += chunk[sg * WPT + wg];
+    }
+
     // 3. first transform G = FFT(F): first pass on the registers, last pass
     // back into registers; P = |G|^2 goes to LDS as z(n) = P(2n) + i P(2n+1)
     A0::compute_regs(v0);
@@ -301,7 +358,7 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
         }
     }
     __syncthreads();
-    // 4-5. prefix sums psum(t + 1) of |F(t) - mean|^2 from a second
+    // 4-5. (not REG) prefix sums psum(t + 1) of |F(t) - mean|^2 from a second
     // (cache-resident) read of the series into the buffer half the second
     // transform leaves free, first and last kEdge samples into `edge`; and the
     // second transform at half length, Z = FFT_{N/2}(z).  PP (the paired
@@ -353,7 +410,16 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
       __syncthreads();
     };
     Acc* const tail = reinterpret_cast<Acc*>(ZA ? buf + padi(NH) : buf);
-    if constexpr (PP) {
+    if constexpr (Lay::REG) {
+      // tail[t] = psum(t + 1), t < L - 1, into the second half, read after the
+      // in-place transform's barriers
+#pragma unroll
+      for (int sg = 0; sg < S; ++sg) {
+        const int t = sg * TPT + u;
+        if (t < L - 1) tail[t] = ps[sg];
+      }
+      p2_passes<T, LOGN, true, 0, NPH>(buf, tw, u);
+    } else if constexpr (PP) {
       p2_half_pp<T, LOGN, 0, NPH>(buf, buf + padi(NH), tw, u);
       prefix_sums(tail);
     } else {
@@ -373,9 +439,13 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
         const Acc* const htail = reinterpret_cast<const Acc*>(ZA ? hgrp + padi(NH) : hgrp);
         const C* const hedge = edge0 + h * 2 * kEdge;
         bool flag = false;  // some lag of this q needs the direct recomputation
-        Acc hptot = 0;
+        Acc hptot = 0;  // psum(L)
+        if constexpr (Lay::REG) {
+          hptot = chunk0[h * Lay::CH + S * WPT];
+        } else {
 #pragma unroll
-        for (int w = 0; w < WPT; ++w) hptot += chunk0[h * WPT + w];
+          for (int w = 0; w < WPT; ++w) hptot += chunk0[h * Lay::CH + w];
+        }
         // This thread's lag iterations k (index nl - 1 - v - k STEP, at most KL
         // of them since nl < N/2).  On the plans with registers to spare (the
         // paired and two-column ones, and FP64, whose LDS allows at most 4
@@ -474,8 +544,12 @@ __global__ void __launch_bounds__((P2Lds<LOGN, T, SUB>::NT), (P2Lds<LOGN, T, SUB
     }
     if (active && u == 0) {
       Acc ptot = 0;
+      if constexpr (Lay::REG) {
+        ptot = chunk[S * WPT];
+      } else {
 #pragma unroll
-      for (int w = 0; w < WPT; ++w) ptot += chunk[w];
+        for (int w = 0; w < WPT; ++w) ptot += chunk[w];
+      }
       const size_t oq = col_out(a, c);
       const Acc mr2 = static_cast<Acc>(fsr / L), mi2 = static_cast<Acc>(fsi / L);
       Acc ps, var;

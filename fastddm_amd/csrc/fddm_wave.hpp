@@ -45,6 +45,15 @@ __device__ __forceinline__ A incl_scan(A v) {
   return v;
 }
 
+// Lane 63's value, in every lane.
+__device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+__device__ __forceinline__ double lane63(double v) {
+  const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u & 0xffffffffull), 63));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), 63));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 // The previous lane's value (0 in lane 0).
 template <typename A>
 __device__ __forceinline__ A from_prev_lane(A v) {
