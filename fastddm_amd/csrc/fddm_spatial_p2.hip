@@ -60,6 +60,22 @@ __device__ __forceinline__ T pixel_value(Tin v, Tin v0, bool sub, T w) {
   }
 }
 
+// Split of pair row p at frequency k, Z = FFT(frame 2p + i frame 2p+1):
+// A(k) = (Z(k) + conj Z(-k)) / 2 and B(k) = (Z(k) - conj Z(-k)) / 2i (times
+// the norm), stored together at dst (frames 2p and 2p+1 are neighbours in R):
+// one 16-byte store in FP32, and one read of the two Z values per pair.
+template <typename T>
+__device__ __forceinline__ void split_store(cplx_t<T>* dst, cplx_t<T> zk, cplx_t<T> zn, T half_norm) {
+  const T ax = (zk.x + zn.x) * half_norm, ay = (zk.y - zn.y) * half_norm;
+  const T bx = (zk.y + zn.y) * half_norm, by = (zn.x - zk.x) * half_norm;
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(dst) = make_float4(ax, ay, bx, by);
+  } else {
+    dst[0] = cplx_t<T>{ax, ay};
+    dst[1] = cplx_t<T>{bx, by};
+  }
+}
+
 template <typename Tin, typename T, int LOGX, int NP>
 __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems) {
   using C = cplx_t<T>;
@@ -135,30 +151,27 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems
   const T half_norm = static_cast<T>(0.5 * a.norm);
   C* __restrict__ R = static_cast<C*>(a.R) + (tb * NXH * static_cast<size_t>(H) + y) * TB;
   const uint32_t kstride = static_cast<uint32_t>(H) * TB;
-  for (int w = tid; w < NXH * TB; w += kThreads) {
-    const int ti = w & (TB - 1), k = w / TB;
-    const int p = ti >> 1;
-    const C zk = buf[padi(p * NX + k)];
-    const C zn = buf[padi(p * NX + ((NX - k) & (NX - 1)))];
-    const C o = (ti & 1) == 0 ? C{(zk.x + zn.x) * half_norm, (zk.y - zn.y) * half_norm}
-                              : C{(zk.y + zn.y) * half_norm, (zn.x - zk.x) * half_norm};
-    R[static_cast<uint32_t>(k) * kstride + ti] = o;
+  for (int w = tid; w < NXH * NP; w += kThreads) {
+    const int p = w & (NP - 1), k = w / NP;
+    split_store<T>(R + static_cast<uint32_t>(k) * kstride + 2 * p, buf[padi(p * NX + k)],
+                   buf[padi(p * NX + ((NX - k) & (NX - 1)))], half_norm);
   }
 }
 
 // Full-width rows of 2-byte pixels (W == NX <= 512), FP32: a persistent form
 // of k_rows_p2 with one wave per pair row.  Workgroup b takes items b,
 // b + grid, ... (item = tb * H + y); wave p loads, converts and transforms
-// pair row p in its own LDS region with no workgroup barrier, loading the
-// next item's pixels, frame-0 row and window values into registers before the
-// transform so their latency overlaps it.  Two barriers per item remain:
-// before the split into half spectra (which reads every pair row) and before
-// the next item rewrites the rows.  The twiddles of passes 1 and 2 are copied
-// into LDS once per workgroup: read from global memory inside the passes,
-// their waits (vmcnt, which also counts the prefetch loads and the previous
-// item's stores) would hold each pass until those had completed.
+// pair row p in its own LDS region with no workgroup barrier (the 32 waves
+// of the four resident workgroups per CU overlap one another's loads).  Two
+// barriers per item remain: before the split into half spectra (which reads
+// every pair row) and before the next item rewrites the rows.  The twiddles
+// of passes 1 and 2 are copied into LDS once per workgroup.  Loads are not
+// carried across items: holding the next item's pixels through the
+// transform needed more than the 64 VGPRs of 8 waves per SIMD, and the
+// spill reloads waited (vmcnt) for every load and store in flight
+// (0.73 ms at 512^2 x 2000 with that prefetch, 0.68-0.69 ms without).
 template <typename Tin, typename T, int LOGX, int NP>
-__global__ void __launch_bounds__(64 * NP, 8) k_rows_p2_pf(RowArgs a, uint32_t nitems) {
+__global__ void __launch_bounds__(64 * NP, 8) k_rows_p2_persist(RowArgs a, uint32_t nitems) {
   using C = cplx_t<T>;
   constexpr int NT = 64 * NP;
   constexpr int NX = 1 << LOGX, NXH = NX / 2 + 1, TB = 2 * NP;
@@ -209,9 +222,9 @@ __global__ void __launch_bounds__(64 * NP, 8) k_rows_p2_pf(RowArgs a, uint32_t n
   };
 
   uint32_t item = blockIdx.x;
-  if (item < nitems) load(item);
   __syncthreads();  // twiddles in LDS
   for (; item < nitems; item += gridDim.x) {
+    load(item);
     const int y = static_cast<int>(item % H);
     const uint64_t tb = item / H, ta = tb * TB + 2 * p;
     const bool has_a = ta < a.frames, has_b = ta + 1 < a.frames;  // uniform over the wave
@@ -229,20 +242,15 @@ __global__ void __launch_bounds__(64 * NP, 8) k_rows_p2_pf(RowArgs a, uint32_t n
       }
     }
     wave_lds_sync();
-    if (item + gridDim.x < nitems) load(item + gridDim.x);
     fft3_wave<T, NX, P.r0, P.r1, P.r2>(row, twl, lane);
     __syncthreads();  // every pair row transformed
 
     // split into the two half spectra and store R[tb][k][y][ti] (ti fastest)
     C* __restrict__ R = static_cast<C*>(a.R) + (tb * NXH * static_cast<size_t>(H) + y) * TB;
-    for (int w = tid; w < NXH * TB; w += NT) {
-      const int ti = w & (TB - 1), k = w / TB;
-      const int pp = ti >> 1;
-      const C zk = buf[padi(pp * NX + k)];
-      const C zn = buf[padi(pp * NX + ((NX - k) & (NX - 1)))];
-      const C o = (ti & 1) == 0 ? C{(zk.x + zn.x) * half_norm, (zk.y - zn.y) * half_norm}
-                                : C{(zk.y + zn.y) * half_norm, (zn.x - zk.x) * half_norm};
-      R[static_cast<uint32_t>(k) * kstride + ti] = o;
+    for (int w = tid; w < NXH * NP; w += NT) {
+      const int pp = w & (NP - 1), k = w / NP;
+      split_store<T>(R + static_cast<uint32_t>(k) * kstride + 2 * pp, buf[padi(pp * NX + k)],
+                     buf[padi(pp * NX + ((NX - k) & (NX - 1)))], half_norm);
     }
     __syncthreads();  // the LDS rows are rewritten by the next item
   }
@@ -332,14 +340,13 @@ template <int LOGX, typename Tin, typename T, int NP>
 hipError_t rows_np(const RowArgs& a, unsigned grid, hipStream_t s) {
   const size_t lds_rows = sizeof(cplx_t<T>) * lds_padded_size(NP * (1 << LOGX));
   const size_t lds = lds_rows;
-  // persistent form (FP32, 2-byte pixels, rows up to 512 wide: the cases whose
-  // prefetch registers fit 8 waves per SIMD without spills); as many
+  // persistent form (FP32, 2-byte pixels, full rows up to 512 wide); as many
   // workgroups as stay resident
   if constexpr (sizeof(Tin) == 2 && sizeof(T) == 4 && LOGX <= 9) {
     if (a.W == (1 << LOGX)) {
       constexpr SpPlan P = sp_plan(LOGX);
       const size_t lds = lds_rows + sizeof(cplx_t<T>) * P.r0 * P.r1 * (1 + P.r2);  // + the LDS twiddles
-      auto kernel = k_rows_p2_pf<Tin, T, LOGX, NP>;
+      auto kernel = k_rows_p2_persist<Tin, T, LOGX, NP>;
       if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
