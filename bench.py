@@ -583,9 +583,10 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
-    ap.add_argument("--skip-large", action="store_true", help="skip the 1024^2 x 4000 leg")
+    ap.add_argument("--skip-large", action="store_true", help="skip the 1024^2 x 4000 and 48-stack batch legs")
     ap.add_argument("--skip-diff-leg", action="store_true", help="skip the diff-mode leg")
     ap.add_argument("--skip-host", action="store_true", help="skip the host-to-host and streamed-ingest legs")
+    ap.add_argument("--skip-batch", action="store_true", help="skip the 48-stack batch leg")
     ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving measurement")
     ap.add_argument("--precision-report", action="store_true",
@@ -714,6 +715,19 @@ def main():
             ingest_parts.append({"begin_ms": (t1 - t0) * 1e3, "push_ms": (t2 - t1) * 1e3, "finish_ms": (t3 - t2) * 1e3,
                                  "close_ms": (ingest_s[-1] - (t3 - t0)) * 1e3})
         log(f"streamed ingest fp32c: {min(ingest_s) * 1e3:.1f} ms per analysis")
+        # the same with a result array whose pages are already mapped (the
+        # difference is the first-touch cost of a fresh 2 GB host array)
+        ing = core.Ingest(mode, img.dtype, img.shape, lags, nx, ny, 0 if diff else nt, win, "fp32c")
+        dst = np.zeros(ing.out_shape, dtype=ing.out_dtype)
+        t0 = time.perf_counter()
+        for k in range(0, L, 64):
+            ing.push(img[k:k + 64])
+        t1 = time.perf_counter()
+        ing.finish(out=dst)
+        ingest_warm = {"push_ms": (t1 - t0) * 1e3, "finish_ms": (time.perf_counter() - t1) * 1e3}
+        ing.close()
+        del dst
+        log(f"streamed ingest into a mapped result: {ingest_warm}")
     fp64 = None
     b64 = algorithmic_bytes(L, S, S, img.itemsize, nq, len(lags), 16, 8, win is not None)
     if not args.skip_fp64:
@@ -791,6 +805,7 @@ def main():
             "frames_per_s": L / min(ingest_s),
             "ms_per_analysis": min(ingest_s) * 1e3,
             "phases_ms": {k: round(v, 2) for k, v in ingest_parts[int(np.argmin(ingest_s))].items()},
+            "into_mapped_result_ms": {k: round(v, 2) for k, v in ingest_warm.items()},
             "note": ("host frames pushed 64 at a time into the streamed ingest (device buffers allocated "
                      "per analysis, included), result copied back to a host array"),
         },
@@ -818,6 +833,15 @@ def main():
         log(f"large leg: {time.perf_counter() - t0:.1f} s")
     if not diff and not args.skip_diff_leg and args.frames is None and args.size is None:
         result["diff_300x256x256"] = diff_leg(core, args)
+    if not diff and not args.skip_large and not args.skip_batch and args.frames is None and args.size is None:
+        t0 = time.perf_counter()
+        b = batch_main(argparse.Namespace(steps=5, warmup=2))
+        result["batch_48_stacks"] = {
+            "workload": b["config"]["workload"], "frames": b["config"]["frames"],
+            "graph_ms": b["ms_per_step"], "eager_ms": b["eager_ms_per_step"], "frames_per_s": b["value"],
+            "graph_speedup_vs_eager": b["graph_speedup_vs_eager"],
+            "frames_per_block_by_shape": b["frames_per_block_by_shape"]}
+        log(f"batch leg: {time.perf_counter() - t0:.1f} s")
     if not args.skip_cpu and not diff:
         t0 = time.perf_counter()
         result["cpu_baseline"] = cpu_baselines(img, lags, nx, ny, nt, win, L, nq)

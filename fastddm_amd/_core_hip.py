@@ -468,8 +468,14 @@ class Ingest:
             raise RuntimeError(f"frames of shape {a.shape[1:]} pushed into a {self.shape[1:]} stack")
         _call("fddm_ingest_push", self._h, a.ctypes.data, a.shape[0])
 
-    def finish(self) -> np.ndarray:
-        out = np.empty(self.out_shape, dtype=self.out_dtype)
+    def finish(self, out: np.ndarray = None) -> np.ndarray:
+        """The result, written into `out` when given (C-contiguous, of the
+        result's shape and dtype), else into a new array."""
+        if out is None:
+            out = np.empty(self.out_shape, dtype=self.out_dtype)
+        elif (out.shape != self.out_shape or out.dtype != self.out_dtype or not out.flags.c_contiguous
+              or not out.flags.writeable):
+            raise RuntimeError(f"output array must be a writeable C-contiguous {self.out_shape} {self.out_dtype} array")
         _call("fddm_ingest_finish", self._h, out.ctypes.data)
         return out
 

@@ -78,3 +78,19 @@ def test_ddm_files_equals_ddm_on_array(hip, confocal, tmp_path):
         assert np.array_equal(got._data, ref._data)
         assert np.array_equal(got.kx, ref.kx) and np.array_equal(got.ky, ref.ky)
         assert np.array_equal(got.tau, ref.tau)
+
+
+def test_ingest_into_given_array(hip):
+    """finish(out=...) writes the result into the caller's array (and refuses
+    one of the wrong shape or dtype before touching the device)."""
+    img = ar1_speckle(40, 32, 32, seed=9)
+    lags = np.arange(1, 40)
+    ref = hip.ddm_fft(img, lags, 32, 32, 80, None, precision="fp32c")
+    ing = hip.Ingest("fft", img.dtype, img.shape, lags, 32, 32, 80, None, "fp32c")
+    ing.push(img)
+    with pytest.raises(RuntimeError, match="output array"):
+        ing.finish(out=np.zeros((3, 32, 17)))
+    dst = np.full(ing.out_shape, np.nan)
+    got = ing.finish(out=dst)
+    ing.close()
+    assert got is dst and np.array_equal(dst, ref)
