@@ -583,7 +583,7 @@ struct HostPathCache {
 // pages are touched by several threads at once.
 void fddm::parallel_memcpy(void* dst, const void* src, size_t n) {
   const size_t per = size_t(4) << 20;
-  unsigned nt = static_cast<unsigned>(std::min<size_t>((n + per - 1) / per, 8));
+  unsigned nt = static_cast<unsigned>(std::min<size_t>((n + per - 1) / per, 16));
   const unsigned hw = std::thread::hardware_concurrency();
   if (hw && nt > hw) nt = hw;
   if (nt <= 1) {
@@ -1232,6 +1232,7 @@ int fddm_release_cache(char* err, size_t errlen) {
   if (c.stream) (void)hipStreamDestroy(c.stream);
   c = HostPathCache{};
   azavg_release_scratch(dev);
+  ingest_release_pool(dev);
   return FDDM_OK;
 }
 

@@ -15,6 +15,7 @@
 // plane and copies the result to the host.
 #include <algorithm>
 #include <cstdarg>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -37,6 +38,7 @@ struct fddm_ingest {
   uint64_t issued = 0;   // frames whose chunk has been handed to the device
   int cur = 0;           // pinned / device buffer being filled
   void* pin[2] = {nullptr, nullptr};
+  bool pooled = false;  // pin[] belong to the device's pool (returned, not freed, at the end)
   void* din[2] = {nullptr, nullptr};
   void* frame0 = nullptr;
   void* edge_in = nullptr;
@@ -70,11 +72,58 @@ int set_err(char* err, size_t len, int code, const char* fmt, ...) {
   } while (0)
 
 constexpr uint64_t kChunkBytes = uint64_t(64) << 20;     // input bytes per chunk (target)
+
+// Pinned chunk buffers kept per device between ingests: buffers pinned anew
+// for every ingest measured slower for the result's device-to-host copies
+// (about 30 GB/s from the second ingest of a process on, against 45 GB/s),
+// so one ingest at a time per device reuses the pool; a concurrent one pins
+// its own.
+struct PinPool {
+  std::mutex mu;
+  bool busy = false;
+  void* buf[2] = {nullptr, nullptr};
+  uint64_t cap = 0;
+  // the workspace and result buffers of the last ingest, kept while they
+  // total at most kKeepDeviceBytes
+  void* ws = nullptr;
+  void* out = nullptr;
+  uint64_t ws_cap = 0, out_cap = 0;
+};
+PinPool g_pool[64];
+constexpr uint64_t kKeepDeviceBytes = uint64_t(16) << 30;
+
+// Frees the pool's device buffers (pool lock held).
+void drop_device(PinPool& pool) {
+  for (void** b : {&pool.ws, &pool.out})
+    if (*b) (void)hipFree(*b);
+  pool.ws = pool.out = nullptr;
+  pool.ws_cap = pool.out_cap = 0;
+}
+
+// Device buffer of at least `need` bytes in *buf (capacity *cap), grown by
+// a new allocation when smaller.
+hipError_t grow_device(void** buf, uint64_t* cap, uint64_t need) {
+  if (*buf && *cap >= need) return hipSuccess;
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *cap = 0;
+  const hipError_t e = hipMalloc(buf, need ? need : 1);
+  if (e == hipSuccess) *cap = need;
+  return e;
+}
 constexpr uint64_t kMinStageBytes = uint64_t(4) << 20;   // smaller chunk buffers: result copied directly
 
 void free_all(fddm_ingest* h) {
   if (h->comp) (void)hipStreamSynchronize(h->comp);
   if (h->copy) (void)hipStreamSynchronize(h->copy);
+  if (h->pooled) {
+    PinPool& pool = g_pool[h->device];
+    std::lock_guard<std::mutex> lk(pool.mu);
+    pool.busy = false;
+    h->pin[0] = h->pin[1] = nullptr;
+    h->ws = h->out = nullptr;
+    if (pool.ws_cap + pool.out_cap > kKeepDeviceBytes) drop_device(pool);
+  }
   for (int k = 0; k < 2; ++k) {
     if (h->pin[k]) (void)hipHostFree(h->pin[k]);
     if (h->din[k]) (void)hipFree(h->din[k]);
@@ -129,6 +178,18 @@ int issue_chunk(fddm_ingest* h, char* err, size_t errlen) {
 
 }  // namespace
 
+void fddm::ingest_release_pool(int dev) {
+  if (dev < 0 || dev >= 64) return;
+  PinPool& pool = g_pool[dev];
+  std::lock_guard<std::mutex> lk(pool.mu);
+  if (pool.busy) return;  // an ingest still uses it; freed by a later release
+  for (void*& b : pool.buf)
+    if (b) (void)hipHostFree(b);
+  pool.buf[0] = pool.buf[1] = nullptr;
+  pool.cap = 0;
+  drop_device(pool);
+}
+
 extern "C" {
 
 int fddm_ingest_begin(fddm_ingest** out, int mode, int dtype, uint64_t length, uint64_t height, uint64_t width,
@@ -152,6 +213,7 @@ int fddm_ingest_begin(fddm_ingest** out, int mode, int dtype, uint64_t length, u
                           precision, out_dtype, 0, err, errlen);
   if (rc) return rc;
   FDDM_HIP(hipGetDevice(&h->device));
+  if (h->device < 0 || h->device >= 64) return set_err(err, errlen, FDDM_ERR_DEVICE, "device id %d not supported", h->device);
   h->info = plan_info(h->plan);
   h->frame_b = height * width * h->info.in_bytes;
   // chunks of whole frame blocks, about kChunkBytes of input each
@@ -160,17 +222,45 @@ int fddm_ingest_begin(fddm_ingest** out, int mode, int dtype, uint64_t length, u
   h->chunk_frames = std::min(h->chunk_frames, (length + tb - 1) / tb * tb);
   FDDM_HIP(hipStreamCreateWithFlags(&h->copy, hipStreamNonBlocking));
   FDDM_HIP(hipStreamCreateWithFlags(&h->comp, hipStreamNonBlocking));
+  const uint64_t pin_b = h->chunk_frames * h->frame_b;
+  {
+    PinPool& pool = g_pool[h->device];
+    std::lock_guard<std::mutex> lk(pool.mu);
+    if (!pool.busy) {
+      if (pool.cap < pin_b) {
+        for (void*& b : pool.buf)
+          if (b) (void)hipHostFree(b);
+        pool.buf[0] = pool.buf[1] = nullptr;
+        pool.cap = 0;
+        for (void*& b : pool.buf) FDDM_HIP(hipHostMalloc(&b, pin_b, hipHostMallocDefault));
+        pool.cap = pin_b;
+      }
+      pool.busy = true;
+      h->pooled = true;
+      h->pin[0] = pool.buf[0];
+      h->pin[1] = pool.buf[1];
+    }
+  }
   for (int k = 0; k < 2; ++k) {
-    FDDM_HIP(hipHostMalloc(&h->pin[k], h->chunk_frames * h->frame_b, hipHostMallocDefault));
-    FDDM_HIP(hipMalloc(&h->din[k], h->chunk_frames * h->frame_b));
+    if (!h->pooled) FDDM_HIP(hipHostMalloc(&h->pin[k], pin_b, hipHostMallocDefault));
+    FDDM_HIP(hipMalloc(&h->din[k], pin_b));
     FDDM_HIP(hipEventCreateWithFlags(&h->copied[k], hipEventDisableTiming));
     FDDM_HIP(hipEventCreateWithFlags(&h->done[k], hipEventDisableTiming));
     FDDM_HIP(hipEventRecord(h->done[k], h->comp));
   }
   FDDM_HIP(hipMalloc(&h->frame0, h->frame_b));
   if (h->info.edge_frames) FDDM_HIP(hipMalloc(&h->edge_in, h->info.edge_frames * h->frame_b));
-  FDDM_HIP(hipMalloc(&h->ws, h->info.ws_bytes));
-  FDDM_HIP(hipMalloc(&h->out, h->info.out_bytes));
+  if (h->pooled) {
+    PinPool& pool = g_pool[h->device];
+    std::lock_guard<std::mutex> lk(pool.mu);
+    FDDM_HIP(grow_device(&pool.ws, &pool.ws_cap, h->info.ws_bytes));
+    FDDM_HIP(grow_device(&pool.out, &pool.out_cap, h->info.out_bytes));
+    h->ws = pool.ws;
+    h->out = pool.out;
+  } else {
+    FDDM_HIP(hipMalloc(&h->ws, h->info.ws_bytes));
+    FDDM_HIP(hipMalloc(&h->out, h->info.out_bytes));
+  }
   guard.keep = true;
   *out = h;
   return FDDM_OK;

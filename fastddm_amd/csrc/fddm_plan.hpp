@@ -42,10 +42,13 @@ int plan_spatial_frames(fddm_plan* p, const void* img, const void* img0, uint64_
                         hipStream_t s, char* err, size_t errlen);
 int plan_finish_from_spectra(fddm_plan* p, const void* frame0, const void* edge_in, void* ws, void* d_out,
                              hipStream_t s, char* err, size_t errlen);
-// memcpy split over up to 8 host threads (4 MiB or more each): fills and
+// memcpy split over up to 16 host threads (4 MiB or more each): fills and
 // drains pinned staging buffers at host-memory speed, and first-touches a
 // fresh destination from several threads.
 void parallel_memcpy(void* dst, const void* src, size_t n);
+// Frees the pinned chunk buffers the streamed ingest keeps for device dev
+// (unless an ingest is using them).
+void ingest_release_pool(int dev);
 // Frees the azimuthal-average scratch kept for device dev.
 void azavg_release_scratch(int dev);
 
