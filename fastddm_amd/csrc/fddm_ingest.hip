@@ -73,11 +73,10 @@ int set_err(char* err, size_t len, int code, const char* fmt, ...) {
 
 constexpr uint64_t kChunkBytes = uint64_t(64) << 20;     // input bytes per chunk (target)
 
-// Pinned chunk buffers kept per device between ingests: buffers pinned anew
-// for every ingest measured slower for the result's device-to-host copies
-// (about 30 GB/s from the second ingest of a process on, against 45 GB/s),
-// so one ingest at a time per device reuses the pool; a concurrent one pins
-// its own.
+// Pinned chunk buffers kept per device between ingests, until the process
+// ends: host memory freed and pinned again measured slower for the result's
+// device-to-host copies (about 30 GB/s against 45 GB/s), so one ingest at a
+// time per device reuses the pool; a concurrent one pins its own.
 struct PinPool {
   std::mutex mu;
   bool busy = false;
@@ -183,11 +182,7 @@ void fddm::ingest_release_pool(int dev) {
   PinPool& pool = g_pool[dev];
   std::lock_guard<std::mutex> lk(pool.mu);
   if (pool.busy) return;  // an ingest still uses it; freed by a later release
-  for (void*& b : pool.buf)
-    if (b) (void)hipHostFree(b);
-  pool.buf[0] = pool.buf[1] = nullptr;
-  pool.cap = 0;
-  drop_device(pool);
+  drop_device(pool);       // the pinned buffers stay (see PinPool)
 }
 
 extern "C" {

@@ -102,8 +102,11 @@ int fddm_ddm_diff(const void *img, int dtype, uint64_t length, uint64_t height, 
                   const void *window, int window_f32, int precision, int out_dtype, void *out,
                   char *err, size_t errlen);
 /* Drops the device buffers cached by fddm_ddm_fft / fddm_ddm_diff, the
- * scratch kept by fddm_azimuthal_average and the pinned chunk buffers kept by
- * the streamed ingest (when no ingest is using them), on the current device. */
+ * scratch kept by fddm_azimuthal_average and the device buffers kept by the
+ * streamed ingest (when no ingest is using them), on the current device.  The
+ * pinned host staging buffers of those calls (2 x 64 MiB each, at least) stay
+ * until the process ends: host memory freed and pinned again measured slower
+ * for device-to-host copies. */
 int fddm_release_cache(char *err, size_t errlen);
 /* Device bytes of input, output and workspace buffers that fddm_ddm_fft /
  * fddm_ddm_diff keep on the current device for the next call (a negative
@@ -144,10 +147,11 @@ int fddm_get_last_call_chunks(uint64_t *stats, char *err, size_t errlen);
  * mode); push takes `count` frames of height*width pixels in stack order;
  * finish, once every frame is pushed, runs the remaining stages and writes
  * the (num_lags+2, ny, nx/2+1) result to host memory -- the same values as
- * fddm_ddm_fft / fddm_ddm_diff.  Device memory is held until destroy; the
- * two pinned chunk buffers come from a per-device pool that the next ingest
- * on the device reuses (a concurrent ingest pins its own), freed by
- * fddm_release_cache. */
+ * fddm_ddm_fft / fddm_ddm_diff.  The two pinned chunk buffers and the
+ * workspace and result device buffers come from a per-device pool that the
+ * next ingest on the device reuses (a concurrent ingest allocates its own);
+ * fddm_release_cache frees the pool's device buffers, and device buffers
+ * above 16 GiB in total are freed at destroy. */
 typedef struct fddm_ingest fddm_ingest;
 int fddm_ingest_begin(fddm_ingest **ingest, int mode, int dtype, uint64_t length, uint64_t height, uint64_t width,
                       const uint32_t *lags, uint64_t num_lags, uint64_t nx, uint64_t ny, uint64_t nt,
