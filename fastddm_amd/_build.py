@@ -11,7 +11,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["fddm_kernels.hip", "fddm_time_p2.hip", "fddm_spatial_p2.hip", "fddm_capi.hip", "fddm_server.hip",
+SOURCES = ["fddm_kernels.hip", "fddm_time_p2.hip", "fddm_spatial_p2.hip", "fddm_spatial_mr.hip", "fddm_capi.hip", "fddm_server.hip",
            "fddm_batch.hip", "fddm_azavg.hip", "fddm_edge.hip",
            "fddm_time_long.hip", "fddm_ingest.hip"]
 TARGET = os.path.join(HERE, "libfddm_hip.so")

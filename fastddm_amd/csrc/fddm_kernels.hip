@@ -340,6 +340,7 @@ int rc_bucket(long long work) {
 }  // namespace
 
 hipError_t launch_rows(const RowArgs& a, int precision, hipStream_t s) {
+  if (rows_mr_ok(a, precision)) return launch_rows_mr(a, precision, s);
   const uint64_t nblocks = (a.frames + a.Tb - 1) / a.Tb;
   const unsigned grid = static_cast<unsigned>(nblocks * a.H);
   const int e = rc_bucket(static_cast<long long>(a.G) * a.nx);
