@@ -54,8 +54,10 @@ __device__ __forceinline__ T load_pixel(const Tin* __restrict__ img, const Tin* 
     if (sub) v -= static_cast<double>(ref[idx0]);
     return static_cast<T>(v);
   } else {
-    Tin v = img[idx];
-    if (sub) v -= ref[idx0];
+    // floating pixels: the difference in the wider of the two types
+    using Wd = std::conditional_t<(sizeof(Tin) > sizeof(T)), Tin, T>;
+    Wd v = static_cast<Wd>(img[idx]);
+    if (sub) v -= static_cast<Wd>(ref[idx0]);
     return static_cast<T>(v);
   }
 }
@@ -351,6 +353,7 @@ hipError_t launch_rows(const RowArgs& a, int precision, hipStream_t s) {
 }
 
 hipError_t launch_cols(const ColArgs& a, int precision, hipStream_t s) {
+  if (cols_mr_ok(a, precision)) return launch_cols_mr(a, precision, s);
   const unsigned grid = static_cast<unsigned>(a.nblocks * a.nxh);
   const int e = rc_bucket(static_cast<long long>(a.G) * a.ny);
   if (precision == PREC_FP64) {

@@ -96,10 +96,14 @@ bool p2_spatial_ok(int logn);
 int p2_spatial_plan(int logn, bool cols, int* r0, int* r1, int* r2);
 hipError_t launch_rows_p2(const RowArgs& a, int precision, hipStream_t s);
 hipError_t launch_cols_p2(const ColArgs& a, int precision, hipStream_t s);
-// Mixed-radix row kernel (one wave per frame pair, nx <= 1024, Tb 4, 8 or
+// Mixed-radix row kernel (one wave per frame pair, nx <= 768, Tb 4, 8 or
 // 16); launch_rows uses it where rows_mr_ok holds.
 bool rows_mr_ok(const RowArgs& a, int precision);
 hipError_t launch_rows_mr(const RowArgs& a, int precision, hipStream_t s);
+// Mixed-radix column kernel (one wave per frame of the block, ny <= 768);
+// launch_cols uses it where cols_mr_ok holds.
+bool cols_mr_ok(const ColArgs& a, int precision);
+hipError_t launch_cols_mr(const ColArgs& a, int precision, hipStream_t s);
 hipError_t launch_fft_test(void* data, int n, int batch, const FftDesc& d, const void* tw,
                            int precision, hipStream_t s);
 

@@ -19,7 +19,10 @@ __device__ __forceinline__ T pixel_value(Tin v, Tin v0, bool sub, T w) {
   } else if constexpr (std::is_integral_v<Tin>) {
     return static_cast<T>(static_cast<double>(v) - (sub ? static_cast<double>(v0) : 0.0)) * w;
   } else {
-    return static_cast<T>(sub ? v - v0 : v) * w;
+    // floating pixels: the difference in the wider of the two types (a
+    // float32 difference would round where the FP64 transform is exact)
+    using Wd = std::conditional_t<(sizeof(Tin) > sizeof(T)), Tin, T>;
+    return static_cast<T>(sub ? static_cast<Wd>(v) - static_cast<Wd>(v0) : static_cast<Wd>(v)) * w;
   }
 }
 

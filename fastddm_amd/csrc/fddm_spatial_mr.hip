@@ -215,6 +215,81 @@ __global__ void __launch_bounds__(64 * NP, sizeof(T) == 4 ? 8 : 4) k_rows_mr(Row
   }
 }
 
+// Column pass + q-major store, Tb = TB frames per block, TB waves: one
+// (kx, frame block) item is the slab R[tb][kx][y][ti]; it is loaded with the
+// frame index turned outermost (transform ti is LDS row ti), wave ti
+// transforms row ti with no workgroup barrier, and the store writes
+// S[ky nxh + kx][tb TB + ti] in runs of TB frames (two frames per 16-byte
+// store in FP32).  Items run kx-major (item = kx nblocks + tb), so the frame
+// blocks whose runs share lines are neighbours in one XCD's range.
+template <typename T, int TB>
+__global__ void __launch_bounds__(64 * TB, sizeof(T) == 4 ? 8 : 4) k_cols_mr(ColArgs a, uint32_t nitems, int rs) {
+  using C = cplx_t<T>;
+  constexpr int NT = 64 * TB;
+  constexpr int KC = 4;                                // slab elements per thread and load round
+  constexpr int KL = (kMrMax * TB + NT * KC - 1) / (NT * KC);  // load rounds, at most
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* const buf = reinterpret_cast<C*>(smem);
+  C* const twl = buf + TB * rs;
+  const int tid = threadIdx.x;
+  const int tw_i = __builtin_amdgcn_readfirstlane(tid >> 6);  // this wave's transform (uniform)
+  C* const col = buf + tw_i * rs;
+  const int H = a.H, ny = a.ny, nxh = a.nxh;
+  const int hl = H < ny ? H : ny;  // rows loaded; y in [hl, ny) is zero padding
+  const int nin = hl * TB;
+  const uint32_t nb = static_cast<uint32_t>(a.nblocks);
+  {
+    const C* __restrict__ tw = static_cast<const C*>(a.twy);
+    const int twn = ny > 1 ? ny - 1 : 1;
+    for (int i = tid; i < twn; i += NT) twl[i] = tw[i];
+  }
+  const uint64_t per = (static_cast<uint64_t>(nitems) + 7u) >> 3, x = blockIdx.x & 7u;
+  const uint64_t end = (x + 1u) * per < nitems ? (x + 1u) * per : nitems;
+  for (uint64_t item = x * per + (blockIdx.x >> 3); item < end; item += gridDim.x >> 3) {
+    int tidi = tid;
+    asm volatile("" : "+v"(tidi));
+    const int lane = tidi & 63;
+    const uint32_t kx = static_cast<uint32_t>(item) / nb, tb = static_cast<uint32_t>(item) - kx * nb;
+    const C* __restrict__ src = static_cast<const C*>(a.R) + (static_cast<size_t>(tb) * nxh + kx) * H * TB;
+    __syncthreads();  // the previous item's rows are read; twiddles in LDS
+    // slab -> LDS rows: element e = y TB + ti goes to row ti, slot y; KC loads
+    // per thread (clamped addresses) issued before their LDS writes
+#pragma unroll
+    for (int l = 0; l < KL; ++l) {
+      if (l * NT * KC < nin) {  // uniform
+        C v[KC];
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int e = tidi + (l * KC + c) * NT;
+          v[c] = src[e < nin ? e : nin - 1];
+        }
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int e = tidi + (l * KC + c) * NT;
+          if (e < nin) buf[(e & (TB - 1)) * rs + lds_pad(e / TB)] = v[c];
+        }
+      }
+    }
+    for (int y = hl + lane; y < ny; y += 64) col[lds_pad(y)] = C{0, 0};
+    __syncthreads();
+    fft_wave_mr<T>(col, a.dy, twl, lane);
+    __syncthreads();  // every transform done
+    C* __restrict__ S = static_cast<C*>(a.S) + static_cast<size_t>(kx) * a.Lp + static_cast<size_t>(tb) * TB;
+    constexpr int TH = TB / 2;
+    for (int w = tidi; w < ny * TH; w += NT) {
+      const int tp = w & (TH - 1), ky = w / TH;
+      const C z0 = buf[(2 * tp) * rs + lds_pad(ky)], z1 = buf[(2 * tp + 1) * rs + lds_pad(ky)];
+      C* const dst = S + static_cast<size_t>(ky) * nxh * a.Lp + 2 * tp;
+      if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<float4*>(dst) = make_float4(z0.x, z0.y, z1.x, z1.y);
+      } else {
+        dst[0] = z0;
+        dst[1] = z1;
+      }
+    }
+  }
+}
+
 size_t rows_mr_lds(const RowArgs& a, int precision, int* rs) {
   *rs = lds_padded_size(a.nx);
   const size_t csz = precision == PREC_FP64 ? 16 : 8;
@@ -277,6 +352,42 @@ hipError_t rows_mr_t(const RowArgs& a, uint32_t items, size_t lds, int rs, hipSt
   }
 }
 
+size_t cols_mr_lds(const ColArgs& a, int precision, int* rs) {
+  *rs = lds_padded_size(a.ny);
+  const size_t csz = precision == PREC_FP64 ? 16 : 8;
+  return csz * (static_cast<size_t>(a.Tb) * *rs + std::max(a.ny - 1, 1));
+}
+
+template <typename T, int TB>
+hipError_t cols_mr_tb(const ColArgs& a, uint32_t items, size_t lds, int rs, hipStream_t s) {
+  auto kernel = k_cols_mr<T, TB>;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), 64 * TB, lds);
+  if (e != hipSuccess) return e;
+  const uint32_t resident = static_cast<uint32_t>(std::max(1, cus) * std::max(1, per_cu)) & ~7u;
+  const uint32_t grid = std::max(8u, std::min(resident, (items + 7u) & ~7u));
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(64 * TB), lds, s, a, items, rs);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t cols_mr_t(const ColArgs& a, uint32_t items, size_t lds, int rs, hipStream_t s) {
+  switch (a.Tb) {
+    case 4: return cols_mr_tb<T, 4>(a, items, lds, rs, s);
+    case 8: return cols_mr_tb<T, 8>(a, items, lds, rs, s);
+    case 16: return cols_mr_tb<T, 16>(a, items, lds, rs, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 }  // namespace
 
 bool rows_mr_ok(const RowArgs& a, int precision) {
@@ -294,6 +405,22 @@ hipError_t launch_rows_mr(const RowArgs& a, int precision, hipStream_t s) {
   const size_t lds = rows_mr_lds(a, precision, &rs);
   return precision == PREC_FP64 ? rows_mr_t<double>(a, static_cast<uint32_t>(items), lds, rs, s)
                                 : rows_mr_t<float>(a, static_cast<uint32_t>(items), lds, rs, s);
+}
+
+bool cols_mr_ok(const ColArgs& a, int precision) {
+  if (a.ny < 2 || a.ny > kMrMax || !(a.Tb == 4 || a.Tb == 8 || a.Tb == 16)) return false;
+  int rs = 0;
+  return cols_mr_lds(a, precision, &rs) <= 96 * 1024;
+}
+
+hipError_t launch_cols_mr(const ColArgs& a, int precision, hipStream_t s) {
+  const uint64_t items = a.nblocks * static_cast<uint64_t>(a.nxh);
+  if (items == 0) return hipSuccess;
+  if (items > 0xfffffff0ull) return hipErrorInvalidValue;
+  int rs = 0;
+  const size_t lds = cols_mr_lds(a, precision, &rs);
+  return precision == PREC_FP64 ? cols_mr_t<double>(a, static_cast<uint32_t>(items), lds, rs, s)
+                                : cols_mr_t<float>(a, static_cast<uint32_t>(items), lds, rs, s);
 }
 
 }  // namespace fddm

@@ -128,6 +128,45 @@ def test_persistent_rows_match_oracle(hip, precision, shape, dtype, window):
     d_ws.free()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32c"])
+@pytest.mark.parametrize("shape,dtype,nx,ny,window", [
+    ((70, 150, 200), np.uint16, 200, 150, True),    # 8-byte pixel vectors
+    ((45, 96, 150), np.uint8, 150, 96, False),      # row not a whole number of vectors
+    ((33, 384, 384), np.int16, 384, 384, True),
+    ((17, 480, 640), np.float32, 640, 480, True),   # 2-pixel vectors
+    ((12, 100, 120), np.uint16, 125, 105, True),    # zero padding in x and y
+    ((9, 50, 70), np.int64, 72, 56, False),         # 8-byte pixels: one per load
+    ((8, 700, 36), np.uint16, 36, 700, False),      # 4 frames per block (fp64), 8 (fp32)
+    ((21, 30, 768), np.uint16, 768, 30, True),      # longest row of the mixed-radix kernels
+    ((5, 768, 30), np.int16, 30, 768, False),       # longest column
+    ((100, 480, 640), np.uint16, 640, 480, True),   # more items than resident workgroups
+])
+def test_mixed_radix_spectra_match_oracle(hip, precision, shape, dtype, nx, ny, window):
+    """Non-power-of-two rows and columns up to 768: the one-wave-per-transform
+    mixed-radix kernels, every pixel load form and frame-block size."""
+    L, H, W = shape
+    rng = np.random.default_rng(L + H + W)
+    if np.issubdtype(dtype, np.integer):
+        info = np.iinfo(dtype)
+        img = rng.integers(max(info.min, -3000), min(info.max, 4000), shape, endpoint=True).astype(dtype)
+    else:
+        img = (rng.random(shape) * 4000).astype(dtype)
+    win = rng.random((H, W)) if window else None
+    plan = hip.Plan("fft", img.dtype, shape, [1], nx, ny, next_fast_len(2 * L, "hip"), win, precision)
+    d_img = hip.DeviceBuffer.from_array(img)
+    d_ws = hip.DeviceBuffer(plan.workspace_size())
+    got = plan.spectra(d_img, d_ws)
+    base = img.astype(np.float64)
+    if precision != "fp32":
+        base = base - base[0]
+    ref = onp.spectra(base, nx, ny, win).reshape(L, -1).T
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err < (1e-13 if precision == "fp64" else 2e-6), err
+    plan.destroy()
+    d_img.free()
+    d_ws.free()
+
+
 # ------------------------------------------------ whole path against the oracle
 CASES = [
     # (L, H, W, nx, ny, window, lags)
