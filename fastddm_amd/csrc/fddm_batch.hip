@@ -10,8 +10,8 @@
 //
 // Autotuner: for each distinct (mode, dtype, H, W, nx, ny, precision,
 // window) the candidate frame-block sizes that fit LDS are timed on a sample
-// of the stack's own frames (HIP events over a few executions) and the fastest
-// is cached for the life of the process.
+// of the stack's own frames (the spatial stages' HIP-event times over a few
+// executions) and the fastest is cached for the life of the process.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -52,7 +52,7 @@ struct TuneKey {
 std::mutex g_tune_mu;
 std::map<TuneKey, int> g_tuned;  // frames per block
 
-constexpr uint64_t kTuneFrames = 128;
+constexpr uint64_t kTuneFrames = 256;
 
 // Times one execution of a plan over the first `frames` frames of `it`.
 int time_candidate(const fddm_batch_item& it, uint64_t frames, int tb, hipStream_t s, float* ms, char* err,
@@ -74,18 +74,16 @@ int time_candidate(const fddm_batch_item& it, uint64_t frames, int tb, hipStream
   }
   void* d_ws = buf;
   void* d_out = static_cast<unsigned char*>(buf) + align256(ws);
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-    rc = set_err(err, errlen, FDDM_ERR_HIP, "autotune events");
+  // the spatial stages only (frame-0 spectra, rows, columns): the frame
+  // block size changes nothing else
   if (!rc) rc = fddm_plan_execute(p, it.d_img, d_out, d_ws, s, err, errlen);  // warm-up
-  if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = set_err(err, errlen, FDDM_ERR_HIP, "autotune event record");
+  if (!rc) rc = fddm_plan_timing_begin(p, 3, err, errlen);
   for (int i = 0; i < 3 && !rc; ++i) rc = fddm_plan_execute(p, it.d_img, d_out, d_ws, s, err, errlen);
-  if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-              hipEventElapsedTime(ms, e0, e1) != hipSuccess))
-    rc = set_err(err, errlen, FDDM_ERR_HIP, "autotune timing");
+  double stage[5] = {0, 0, 0, 0, 0};
+  int steps = 0;
+  if (!rc) rc = fddm_plan_timing_end(p, stage, &steps, err, errlen);
+  if (!rc) *ms = static_cast<float>((stage[0] + stage[1] + stage[2]) / std::max(steps, 1));
   (void)hipStreamSynchronize(s);
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
   (void)hipFree(buf);
   fddm_plan_destroy(p);
   return rc;
