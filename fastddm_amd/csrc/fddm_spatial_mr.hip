@@ -1,25 +1,24 @@
-// Stage (a) row pass for transform lengths that are not powers of two
-// (7-smooth nx up to 768, e.g. 96, 150, 200, 384, 480, 640): the data flow
-// of k_rows (fddm_kernels.hip) reorganised the way the power-of-two
-// persistent row kernel is:
+// Stage (a) row and column passes for transform lengths that are not powers
+// of two (7-smooth, up to 2048: 96, 150, 200, 384, 480, 640, 1280, 1920, ...):
+// the data flow of k_rows / k_cols (fddm_kernels.hip) reorganised the way the
+// power-of-two persistent row kernel is:
 //
-//   * one wave per frame pair: wave p converts (+ frame-0 subtraction) x
-//     window the pixels of frames (2p, 2p+1) of one image row into its own LDS
-//     row and transforms it with no workgroup barrier (runtime radix
+//   * one wave per transform (rows: frames 2p and 2p+1 of one image row as
+//     one complex row; columns: one frame of the block), held in the wave's
+//     own LDS row and transformed with no workgroup barrier (runtime radix
 //     sequence 8s, 4s, 2s, 3s, 5s, 7s; each pass reads all of a lane's
 //     butterflies into registers before it writes, and the wave's LDS
 //     operations complete in order, so passes run in place);
 //   * the pass twiddles are copied into LDS once per workgroup, which then
-//     loops over (frame block, image row) items;
+//     loops over items ((frame block, image row) or (kx, frame block));
 //   * pixels and window values come in 8-byte / 16-byte vector loads, all of
-//     a lane's loads issued before the first conversion;
-//   * the split into the two half spectra reads Z(k) and Z(-k) once per pair
-//     and writes both frames' values with one 16-byte store (FP32).
+//     a lane's loads of a round issued before the first conversion;
+//   * 16-byte stores: the row split writes both frames of a pair at once
+//     from one read of Z(k) and Z(-k), the columns two frames of a q run.
 //
 // Items are dealt so that each XCD works through a contiguous range: the
-// runs of neighbouring image rows share 128-byte lines when a frame block
-// holds fewer than 16 FP32 (8 FP64) frames, and those halves then meet in
-// one L2.
+// runs of neighbouring items share 128-byte lines when a frame block holds
+// fewer than 16 FP32 (8 FP64) frames, and those halves then meet in one L2.
 #include <algorithm>
 
 #include "fddm_fft.hpp"
@@ -30,16 +29,36 @@
 namespace fddm {
 namespace {
 
-constexpr int kMrMax = 768;  // longest row of these kernels
+// Longest transforms of the two kernel forms: up to kMrShort points the
+// FP32 kernels fit 64 VGPRs (8 waves per SIMD, the occupancy their LDS
+// allows); longer ones, up to kMrLong, hold fewer workgroups per CU anyway
+// (their LDS rows) and get the registers for it.
+constexpr int kMrShort = 768, kMrLong = 2048;
+template <int KMAX, typename T>
+constexpr int mr_min_waves() {
+  return KMAX <= kMrShort ? (sizeof(T) == 4 ? 8 : 4) : (sizeof(T) == 4 ? 3 : 2);
+}
+
+// LDS slot of element e of a transform row.  FP32 rows are not padded:
+// those kernels are bound by VALU issue, and the pad slot every 32 elements
+// of lds_pad (fddm_fft.hpp) costs three more instructions per access than
+// the bank conflicts of the early passes' strided writes cost (row pass
+// 7.72 -> 7.21 ms, columns 8.76 -> 8.63 ms over the 48-stack batch's
+// shapes).  FP64 rows (16-byte elements) keep the pad (480x640 x 1000: rows
+// 1.15 ms padded, 1.18 ms not; columns 1.40 and 1.48 ms).
+template <typename T>
+__device__ __forceinline__ int mr_pad(int e) {
+  return sizeof(T) == 8 ? e + (e >> 5) : e;
+}
 
 // One radix-R Stockham pass (formulation of fddm_fft.hpp) of a length-n
-// transform held by one wave: element e at row[lds_pad(e)], lane handles
+// transform held by one wave: element e at row[mr_pad<T>(e)], lane handles
 // butterflies j = lane + 64 b.  ns: product of the radices already applied;
 // tw: this pass's table, tw[k (R-1) + r - 1] = exp(-2 pi i r k / (ns R)).
-template <typename T, int R>
+template <typename T, int R, int KMAX>
 __device__ __forceinline__ void wave_pass(cplx_t<T>* row, int n, int ns, const cplx_t<T>* tw, int lane) {
   using C = cplx_t<T>;
-  constexpr int MB = (kMrMax / R + 63) / 64;
+  constexpr int MB = (KMAX / R + 63) / 64;
   // opaque lane index: the lane-derived address arithmetic stays in this
   // pass instead of being hoisted to the kernel entry and kept live
   asm volatile("" : "+v"(lane));
@@ -55,7 +74,7 @@ __device__ __forceinline__ void wave_pass(cplx_t<T>* row, int n, int ns, const c
     const int j = lane + 64 * b;
     if (64 * b < m && j < m) {  // (the first test is uniform)
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = row[lds_pad(j + r * m)];
+      for (int r = 0; r < R; ++r) v[b][r] = row[mr_pad<T>(j + r * m)];
       if (ns > 1) {
         const C* t = tw + k * (R - 1);
 #pragma unroll
@@ -71,7 +90,7 @@ __device__ __forceinline__ void wave_pass(cplx_t<T>* row, int n, int ns, const c
   for (int b = 0; b < MB; ++b) {
     if (ob[b] >= 0) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) row[lds_pad(ob[b] + r * ns)] = v[b][r];
+      for (int r = 0; r < R; ++r) row[mr_pad<T>(ob[b] + r * ns)] = v[b][r];
     }
   }
   p2::wave_lds_sync();
@@ -79,13 +98,13 @@ __device__ __forceinline__ void wave_pass(cplx_t<T>* row, int n, int ns, const c
 
 // Forward FFT (unnormalised) of one length-d.n row by one wave; passes in
 // the order of the twiddle table (8s, 4s, 2s, 3s, 5s, 7s).
-template <typename T>
+template <typename T, int KMAX>
 __device__ __forceinline__ void fft_wave_mr(cplx_t<T>* row, const FftDesc& d, const cplx_t<T>* tw, int lane) {
   const int n = d.n;
   int ns = 1;
 #define FDDM_WAVE_PASSES(RAD, COUNT)                  \
   for (int i = 0; i < d.COUNT; ++i) {                 \
-    wave_pass<T, RAD>(row, n, ns, tw, lane);          \
+    wave_pass<T, RAD, KMAX>(row, n, ns, tw, lane);    \
     tw += ns * (RAD - 1);                             \
     ns *= RAD;                                        \
   }
@@ -101,13 +120,13 @@ __device__ __forceinline__ void fft_wave_mr(cplx_t<T>* row, const FftDesc& d, co
 // Row pass, Tb = 2 NP frames per block, NP waves.  rs: LDS slots per pair
 // row (lds_padded_size(nx)); vec: pixel rows start 8-byte aligned and the
 // loaded width is a multiple of the vector (host-checked).
-template <typename Tin, typename T, int NP>
-__global__ void __launch_bounds__(64 * NP, sizeof(T) == 4 ? 8 : 4) k_rows_mr(RowArgs a, uint32_t nitems, int rs, int vec) {
+template <typename Tin, typename T, int NP, int KMAX>
+__global__ void __launch_bounds__(64 * NP, (mr_min_waves<KMAX, T>())) k_rows_mr(RowArgs a, uint32_t nitems, int rs, int vec) {
   using C = cplx_t<T>;
   constexpr int NT = 64 * NP, TB = 2 * NP;
   constexpr int VX = sizeof(Tin) <= 4 ? 8 / static_cast<int>(sizeof(Tin)) : 1;
   // vectors per lane and frame row, at most (even: taken two at a time)
-  constexpr int KV = ((kMrMax / VX + 63) / 64 + 1) & ~1;
+  constexpr int KV = ((KMAX / VX + 63) / 64 + 1) & ~1;
   struct alignas(8) Vec {
     Tin v[VX];
   };
@@ -157,7 +176,7 @@ __global__ void __launch_bounds__(64 * NP, sizeof(T) == 4 ? 8 : 4) k_rows_mr(Row
         const Tin p0 = sub ? r0[x0] : Tin(0);
         const T fa = has_a ? sp::pixel_value<Tin, T>(ra[x0], p0, sub, w) : T(0);
         const T fb = has_b ? sp::pixel_value<Tin, T>(ra[plane + x0], p0, sub, w) : T(0);
-        row[lds_pad(x0)] = C{fa, fb};
+        row[mr_pad<T>(x0)] = C{fa, fb};
       }
     };
     if constexpr (sizeof(Tin) <= 4) {
@@ -186,7 +205,7 @@ __global__ void __launch_bounds__(64 * NP, sizeof(T) == 4 ? 8 : 4) k_rows_mr(Row
                   const T w = win ? wv[h].v[v] : T(1);
                   const T fa = has_a ? sp::pixel_value<Tin, T>(va[h].v[v], v0[h].v[v], sub, w) : T(0);
                   const T fb = has_b ? sp::pixel_value<Tin, T>(vb[h].v[v], v0[h].v[v], sub, w) : T(0);
-                  row[lds_pad(x0 + v)] = C{fa, fb};
+                  row[mr_pad<T>(x0 + v)] = C{fa, fb};
                 }
               }
             }
@@ -198,9 +217,9 @@ __global__ void __launch_bounds__(64 * NP, sizeof(T) == 4 ? 8 : 4) k_rows_mr(Row
     } else {
       scalar_rows();
     }
-    for (int xp = wl + lane; xp < nx; xp += 64) row[lds_pad(xp)] = C{0, 0};
+    for (int xp = wl + lane; xp < nx; xp += 64) row[mr_pad<T>(xp)] = C{0, 0};
     p2::wave_lds_sync();
-    fft_wave_mr<T>(row, a.dx, twl, lane);
+    fft_wave_mr<T, KMAX>(row, a.dx, twl, lane);
     __syncthreads();  // every pair row transformed
 
     // split into the two half spectra, store R[tb][k][y][ti] (ti fastest)
@@ -208,8 +227,8 @@ __global__ void __launch_bounds__(64 * NP, sizeof(T) == 4 ? 8 : 4) k_rows_mr(Row
     for (int w = tidi; w < nxh * NP; w += NT) {
       const int pp = w & (NP - 1), k = w / NP;
       const C* const rp = buf + pp * rs;
-      sp::split_store<T>(R + static_cast<uint32_t>(k) * kstride + 2 * pp, rp[lds_pad(k)],
-                         rp[lds_pad(k == 0 ? 0 : nx - k)], half_norm);
+      sp::split_store<T>(R + static_cast<uint32_t>(k) * kstride + 2 * pp, rp[mr_pad<T>(k)],
+                         rp[mr_pad<T>(k == 0 ? 0 : nx - k)], half_norm);
     }
     __syncthreads();  // the LDS rows are rewritten by the next item
   }
@@ -222,12 +241,12 @@ __global__ void __launch_bounds__(64 * NP, sizeof(T) == 4 ? 8 : 4) k_rows_mr(Row
 // S[ky nxh + kx][tb TB + ti] in runs of TB frames (two frames per 16-byte
 // store in FP32).  Items run kx-major (item = kx nblocks + tb), so the frame
 // blocks whose runs share lines are neighbours in one XCD's range.
-template <typename T, int TB>
-__global__ void __launch_bounds__(64 * TB, sizeof(T) == 4 ? 8 : 4) k_cols_mr(ColArgs a, uint32_t nitems, int rs) {
+template <typename T, int TB, int KMAX>
+__global__ void __launch_bounds__(64 * TB, (mr_min_waves<KMAX, T>())) k_cols_mr(ColArgs a, uint32_t nitems, int rs) {
   using C = cplx_t<T>;
   constexpr int NT = 64 * TB;
   constexpr int KC = 4;                                // slab elements per thread and load round
-  constexpr int KL = (kMrMax * TB + NT * KC - 1) / (NT * KC);  // load rounds, at most
+  constexpr int KL = (KMAX * TB + NT * KC - 1) / (NT * KC);  // load rounds, at most
   extern __shared__ __align__(16) unsigned char smem[];
   C* const buf = reinterpret_cast<C*>(smem);
   C* const twl = buf + TB * rs;
@@ -266,19 +285,19 @@ __global__ void __launch_bounds__(64 * TB, sizeof(T) == 4 ? 8 : 4) k_cols_mr(Col
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
           const int e = tidi + (l * KC + c) * NT;
-          if (e < nin) buf[(e & (TB - 1)) * rs + lds_pad(e / TB)] = v[c];
+          if (e < nin) buf[(e & (TB - 1)) * rs + mr_pad<T>(e / TB)] = v[c];
         }
       }
     }
-    for (int y = hl + lane; y < ny; y += 64) col[lds_pad(y)] = C{0, 0};
+    for (int y = hl + lane; y < ny; y += 64) col[mr_pad<T>(y)] = C{0, 0};
     __syncthreads();
-    fft_wave_mr<T>(col, a.dy, twl, lane);
+    fft_wave_mr<T, KMAX>(col, a.dy, twl, lane);
     __syncthreads();  // every transform done
     C* __restrict__ S = static_cast<C*>(a.S) + static_cast<size_t>(kx) * a.Lp + static_cast<size_t>(tb) * TB;
     constexpr int TH = TB / 2;
     for (int w = tidi; w < ny * TH; w += NT) {
       const int tp = w & (TH - 1), ky = w / TH;
-      const C z0 = buf[(2 * tp) * rs + lds_pad(ky)], z1 = buf[(2 * tp + 1) * rs + lds_pad(ky)];
+      const C z0 = buf[(2 * tp) * rs + mr_pad<T>(ky)], z1 = buf[(2 * tp + 1) * rs + mr_pad<T>(ky)];
       C* const dst = S + static_cast<size_t>(ky) * nxh * a.Lp + 2 * tp;
       if constexpr (sizeof(T) == 4) {
         *reinterpret_cast<float4*>(dst) = make_float4(z0.x, z0.y, z1.x, z1.y);
@@ -298,7 +317,7 @@ size_t rows_mr_lds(const RowArgs& a, int precision, int* rs) {
 
 template <typename Tin, typename T, int NP>
 hipError_t rows_mr_np(const RowArgs& a, uint32_t items, size_t lds, int rs, int vec, hipStream_t s) {
-  auto kernel = k_rows_mr<Tin, T, NP>;
+  auto kernel = a.nx <= kMrShort ? k_rows_mr<Tin, T, NP, kMrShort> : k_rows_mr<Tin, T, NP, kMrLong>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
@@ -360,7 +379,9 @@ size_t cols_mr_lds(const ColArgs& a, int precision, int* rs) {
 
 template <typename T, int TB>
 hipError_t cols_mr_tb(const ColArgs& a, uint32_t items, size_t lds, int rs, hipStream_t s) {
-  auto kernel = k_cols_mr<T, TB>;
+  // (the long form is FP32 only, see cols_mr_ok)
+  constexpr int KL = sizeof(T) == 4 ? kMrLong : kMrShort;
+  auto kernel = a.ny <= kMrShort ? k_cols_mr<T, TB, kMrShort> : k_cols_mr<T, TB, KL>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
@@ -391,9 +412,14 @@ hipError_t cols_mr_t(const ColArgs& a, uint32_t items, size_t lds, int rs, hipSt
 }  // namespace
 
 bool rows_mr_ok(const RowArgs& a, int precision) {
-  if (a.nx < 2 || a.nx > kMrMax || !(a.Tb == 4 || a.Tb == 8 || a.Tb == 16)) return false;
+  if (a.nx < 2 || a.nx > kMrLong || !(a.Tb == 4 || a.Tb == 8 || a.Tb == 16)) return false;
   int rs = 0;
-  return rows_mr_lds(a, precision, &rs) <= 96 * 1024;
+  const size_t lds = rows_mr_lds(a, precision, &rs);
+  // FP64 rows longer than kMrShort only while two workgroups fit a CU: with
+  // one (two waves) the generic kernel was as fast (1080 x 1920 x 400: 8.35
+  // vs 8.08 ms; 1000^2 x 800, three workgroups: 4.5 vs 6.3 ms)
+  if (precision == PREC_FP64 && a.nx > kMrShort && lds > 80 * 1024) return false;
+  return lds <= 160 * 1024;
 }
 
 hipError_t launch_rows_mr(const RowArgs& a, int precision, hipStream_t s) {
@@ -408,9 +434,12 @@ hipError_t launch_rows_mr(const RowArgs& a, int precision, hipStream_t s) {
 }
 
 bool cols_mr_ok(const ColArgs& a, int precision) {
-  if (a.ny < 2 || a.ny > kMrMax || !(a.Tb == 4 || a.Tb == 8 || a.Tb == 16)) return false;
+  if (a.ny < 2 || a.ny > kMrLong || !(a.Tb == 4 || a.Tb == 8 || a.Tb == 16)) return false;
+  // FP64 columns longer than kMrShort measured slower than the generic
+  // kernel (1000^2 x 800: 8.76 vs 5.85 ms; 1080 x 1920 x 400: 10.2 vs 6.8 ms)
+  if (precision == PREC_FP64 && a.ny > kMrShort) return false;
   int rs = 0;
-  return cols_mr_lds(a, precision, &rs) <= 96 * 1024;
+  return cols_mr_lds(a, precision, &rs) <= 160 * 1024;
 }
 
 hipError_t launch_cols_mr(const ColArgs& a, int precision, hipStream_t s) {

@@ -138,11 +138,17 @@ def test_persistent_rows_match_oracle(hip, precision, shape, dtype, window):
     ((9, 50, 70), np.int64, 72, 56, False),         # 8-byte pixels: one per load
     ((8, 700, 36), np.uint16, 36, 700, False),      # 4 frames per block (fp64), 8 (fp32)
     ((21, 30, 768), np.uint16, 768, 30, True),      # longest row of the mixed-radix kernels
-    ((5, 768, 30), np.int16, 30, 768, False),       # longest column
+    ((5, 768, 30), np.int16, 30, 768, False),       # longest column of the short form
     ((100, 480, 640), np.uint16, 640, 480, True),   # more items than resident workgroups
+    # long form (769 .. 2048 points)
+    ((6, 40, 1280), np.uint16, 1280, 40, True),
+    ((5, 1080, 30), np.uint16, 30, 1080, False),
+    ((4, 64, 1920), np.float32, 1920, 64, True),
+    ((3, 20, 2000), np.int16, 2000, 20, False),
+    ((3, 1000, 1000), np.uint16, 1000, 1000, True),
 ])
 def test_mixed_radix_spectra_match_oracle(hip, precision, shape, dtype, nx, ny, window):
-    """Non-power-of-two rows and columns up to 768: the one-wave-per-transform
+    """Non-power-of-two rows and columns up to 2048: the one-wave-per-transform
     mixed-radix kernels, every pixel load form and frame-block size."""
     L, H, W = shape
     rng = np.random.default_rng(L + H + W)
