@@ -10,8 +10,9 @@
 //
 // Autotuner: for each distinct (mode, dtype, H, W, nx, ny, precision,
 // window) the candidate frame-block sizes that fit LDS are timed on a sample
-// of the stack's own frames (the spatial stages' HIP-event times over a few
-// executions) and the fastest is cached for the life of the process.
+// of the shape's longest stack (the spatial stages' HIP-event times over a
+// few executions, best of two) and the fastest is cached for the life of the
+// process; a smaller block must win by more than 3 %.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -52,7 +53,7 @@ struct TuneKey {
 std::mutex g_tune_mu;
 std::map<TuneKey, int> g_tuned;  // frames per block
 
-constexpr uint64_t kTuneFrames = 256;
+constexpr uint64_t kTuneFrames = 512;
 
 // Times one execution of a plan over the first `frames` frames of `it`.
 int time_candidate(const fddm_batch_item& it, uint64_t frames, int tb, hipStream_t s, float* ms, char* err,
@@ -101,13 +102,16 @@ int tuned_frames_per_block(const fddm_batch_item& it, hipStream_t s, int* tb_out
     }
   }
   const uint64_t frames = std::min<uint64_t>(it.length, kTuneFrames);
+  // largest block first; a smaller one must be faster by more than 3 % (the
+  // best of two timings each), so noise does not pick more, shorter runs
   int best = 0;
   float best_ms = 0;
-  for (int tb : {4, 8, 16}) {
-    float ms = 0;
+  for (int tb : {16, 8, 4}) {
+    float ms = 0, ms2 = 0;
     char e2[256];
     if (time_candidate(it, frames, tb, s, &ms, e2, sizeof e2) != FDDM_OK) continue;  // does not fit: skip
-    if (!best || ms < best_ms) {
+    if (time_candidate(it, frames, tb, s, &ms2, e2, sizeof e2) == FDDM_OK) ms = std::min(ms, ms2);
+    if (!best || ms < 0.97f * best_ms) {
       best = tb;
       best_ms = ms;
     }
@@ -166,15 +170,27 @@ int fddm_batch_create(fddm_batch** out, const fddm_batch_item* items, int n, int
   if (hipStreamCreateWithFlags(&tune_stream, hipStreamNonBlocking) != hipSuccess)
     return fail(set_err(err, errlen, FDDM_ERR_HIP, "stream creation failed"));
   uint64_t total = 0;
+  // each image shape is tuned on its longest stack (a short first stack
+  // would favour small blocks for all of them)
+  std::map<TuneKey, int> longest;
   for (int i = 0; i < n; ++i) {
     const fddm_batch_item& it = items[i];
     if (!it.d_img || !it.d_out) {
       (void)hipStreamDestroy(tune_stream);
       return fail(set_err(err, errlen, FDDM_ERR_ARG, "item %d: NULL device pointer", i));
     }
+    const TuneKey k{it.mode, it.dtype, it.precision, it.out_dtype, it.window != nullptr, it.height, it.width, it.nx,
+                    it.ny};
+    auto f = longest.find(k);
+    if (f == longest.end() || items[f->second].length < it.length) longest[k] = i;
+  }
+  for (int i = 0; i < n; ++i) {
+    const fddm_batch_item& it = items[i];
     int tb = 0;
     if (autotune) {
-      const int rc = tuned_frames_per_block(it, tune_stream, &tb, err, errlen);
+      const TuneKey k{it.mode, it.dtype, it.precision, it.out_dtype, it.window != nullptr, it.height, it.width, it.nx,
+                      it.ny};
+      const int rc = tuned_frames_per_block(items[longest[k]], tune_stream, &tb, err, errlen);
       if (rc) {
         (void)hipStreamDestroy(tune_stream);
         return fail(rc);
