@@ -262,8 +262,9 @@ int fddm_fill_uniform(void *d, uint64_t count, int dtype, uint64_t seed, uint32_
  * analysis with device-resident input and output (output of
  * fddm_plan_output_size bytes for the same arguments).  Creation builds one
  * plan per item -- with autotune != 0 the frames-per-block of each image
- * shape is chosen by timing the candidates on the item's first frames
- * (cached per shape for the process) -- allocates one workspace for all
+ * shape is chosen by timing the candidates' spatial stages on the first
+ * frames of that shape's longest item (cached per shape for the process)
+ * -- allocates one workspace for all
  * items and captures every item's stage chain into one hipGraph on parallel
  * branches.  fddm_batch_launch replays the graph on `stream`;
  * fddm_batch_launch_eager runs the same plans one after another without the
