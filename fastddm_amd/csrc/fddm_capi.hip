@@ -334,7 +334,7 @@ int fddm::plan_create_ex(fddm_plan** out, int mode, int dtype, uint64_t L, uint6
   if (mode == MODE_FFT) {
     // The caller's nt (>= 2L - 1) fixes no result: D is the same for every
     // zero-padded length >= 2L - 1.  The time kernel runs at the power of two
-    // 2^logn_t >= max(2L - 1, 512).
+    // 2^logn_t >= max(2L - 1, 256).
     if (nt < 2 * L - 1) return fail(set_err(err, errlen, FDDM_ERR_ARG, "nt must be >= 2*length-1"));
     int logn = kP2MinLog;
     while ((1ull << logn) < 2 * L - 1) ++logn;

@@ -54,7 +54,8 @@ struct P2Plan {
   int threads, np, r0, r1, r2, r3;
 };
 __host__ __device__ constexpr P2Plan p2_plan(int logn, bool f64) {
-  return logn == 9    ? P2Plan{64, 3, 8, 8, 8, 1}
+  return logn == 8    ? P2Plan{64, 4, 4, 4, 4, 4}
+         : logn == 9  ? P2Plan{64, 3, 8, 8, 8, 1}
          : logn == 10 ? P2Plan{64, 3, 16, 16, 4, 1}
          : logn == 11 ? P2Plan{128, 3, 16, 16, 8, 1}
          : logn == 12 ? (f64 ? P2Plan{256, 3, 16, 16, 16, 1} : P2Plan{512, 4, 8, 8, 8, 8})
@@ -69,7 +70,8 @@ __host__ __device__ constexpr int p2_radix(const P2Plan& P, int i) {
 // same threads (passes with fewer butterflies than threads leave whole waves
 // idle), followed by a per-lag untangling step.
 __host__ __device__ constexpr P2Plan p2_half_plan(int logn, bool f64) {
-  return logn == 9    ? P2Plan{64, 3, 8, 8, 4, 1}
+  return logn == 8    ? P2Plan{64, 4, 4, 4, 4, 2}
+         : logn == 9  ? P2Plan{64, 3, 8, 8, 4, 1}
          : logn == 10 ? P2Plan{64, 3, 8, 8, 8, 1}
          : logn == 11 ? P2Plan{128, 3, 16, 16, 4, 1}
          : logn == 12 ? (f64 ? P2Plan{256, 3, 16, 16, 8, 1} : P2Plan{512, 4, 8, 8, 8, 4})
@@ -94,7 +96,7 @@ __host__ __device__ constexpr int p2_tw_post(int logn, bool f64) {
   return p2_tw_h(logn, f64, p2_half_plan(logn, f64).np);
 }
 __host__ __device__ constexpr int p2_tw_total(int logn, bool f64) { return p2_tw_post(logn, f64) + (1 << (logn - 1)); }
-constexpr int kP2MinLog = 9, kP2MaxLog = 14;
+constexpr int kP2MinLog = 8, kP2MaxLog = 14;
 
 template <typename T> struct Cplx;
 template <> struct Cplx<float> { using type = float2; };

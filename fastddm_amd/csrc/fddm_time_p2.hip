@@ -595,6 +595,7 @@ hipError_t launch_one(const TimeArgs& a, hipStream_t s) {
 template <typename T, typename OutT, bool SUB>
 hipError_t launch_logn(const TimeArgs& a, int logn, hipStream_t s) {
   switch (logn) {
+    case 8: return launch_one<T, OutT, 8, SUB>(a, s);
     case 9: return launch_one<T, OutT, 9, SUB>(a, s);
     case 10: return launch_one<T, OutT, 10, SUB>(a, s);
     case 11: return launch_one<T, OutT, 11, SUB>(a, s);
@@ -614,6 +615,7 @@ size_t p2_lds_bytes(int logn, int precision) {
     using T = decltype(tag);
     constexpr bool S = decltype(sub)::value;
     switch (logn) {
+      case 8: return P2Lds<8, T, S>::bytes;
       case 9: return P2Lds<9, T, S>::bytes;
       case 10: return P2Lds<10, T, S>::bytes;
       case 11: return P2Lds<11, T, S>::bytes;

@@ -204,16 +204,16 @@ def test_whole_path_matches_oracle(hip, mode, case, precision):
     CHECK[precision](out, ref)
 
 
-# Every power-of-two time plan (N = 512 .. 16384: three- and four-pass plans,
+# Every power-of-two time plan (N = 256 .. 16384: three- and four-pass plans,
 # 64 .. 1024 threads per transform) on a small image.
-TIME_LENGTHS = [(200, 512), (300, 1024), (700, 2048), (1500, 4096), (3000, 8192), (5000, 16384),
+TIME_LENGTHS = [(100, 256), (200, 512), (300, 1024), (700, 2048), (1500, 4096), (3000, 8192), (5000, 16384),
                 (10000, 32768)]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32c", "fp32"])
 @pytest.mark.parametrize("L,N", TIME_LENGTHS)
 def test_every_time_plan_matches_oracle(hip, L, N, precision):
-    assert 2 * L - 1 <= N < 4 * L - 2 or N == 512
+    assert 2 * L - 1 <= N < 4 * L - 2 or N == 256
     img = ar1_speckle(L, 12, 10, sigma=1.2, diffusivity=0.05, mean=800, std=120, seed=L)
     lags = np.unique(np.concatenate([np.arange(1, 40), np.geomspace(40, L - 1, 60).astype(int),
                                      np.arange(L - 40, L)]))
