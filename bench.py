@@ -249,6 +249,7 @@ def batch_main(args):
     ms_graph = timed(batch.launch)
     ms_eager = timed(batch.launch_eager)
     tbs = batch.frames_per_block()
+    kinds = batch.spatial_kernels()
     frames = int(Ls.sum())
     batch.close()
     stream.destroy()
@@ -274,6 +275,7 @@ def batch_main(args):
         "eager_ms_per_step": ms_eager,
         "graph_speedup_vs_eager": ms_eager / ms_graph,
         "frames_per_block_by_shape": {f"{h}x{w}": tbs[k] for k, (h, w) in enumerate(shapes)},
+        "mixed_radix_rows_cols_by_shape": {f"{h}x{w}": list(kinds[k]) for k, (h, w) in enumerate(shapes)},
     }
 
 
@@ -840,7 +842,8 @@ def main():
             "workload": b["config"]["workload"], "frames": b["config"]["frames"],
             "graph_ms": b["ms_per_step"], "eager_ms": b["eager_ms_per_step"], "frames_per_s": b["value"],
             "graph_speedup_vs_eager": b["graph_speedup_vs_eager"],
-            "frames_per_block_by_shape": b["frames_per_block_by_shape"]}
+            "frames_per_block_by_shape": b["frames_per_block_by_shape"],
+            "mixed_radix_rows_cols_by_shape": b["mixed_radix_rows_cols_by_shape"]}
         log(f"batch leg: {time.perf_counter() - t0:.1f} s")
     if not args.skip_cpu and not diff:
         t0 = time.perf_counter()

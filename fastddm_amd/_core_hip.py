@@ -103,6 +103,7 @@ def lib() -> ctypes.CDLL:
         "fddm_batch_launch": [p, p, cp, sz],
         "fddm_batch_launch_eager": [p, p, cp, sz],
         "fddm_batch_frames_per_block": [p, ctypes.POINTER(i32)],
+        "fddm_batch_spatial_kernels": [p, ctypes.POINTER(i32), ctypes.POINTER(i32)],
         "fddm_batch_destroy": [p],
         "fddm_azimuthal_average": [p, i32, u64, u64, u64, p, p, p, i32, i32, p, p, p, cp, sz],
     }
@@ -167,6 +168,7 @@ EXPORTED_SYMBOLS = (
     "fddm_batch_launch",
     "fddm_batch_launch_eager",
     "fddm_batch_frames_per_block",
+    "fddm_batch_spatial_kernels",
     "fddm_batch_destroy",
     "fddm_azimuthal_average",
 )
@@ -750,6 +752,13 @@ class Batch:
         v = (ctypes.c_int * self._n)()
         lib().fddm_batch_frames_per_block(self._h, v)
         return list(v)
+
+    def spatial_kernels(self):
+        """Per item, (rows, columns): True where the mixed-radix kernel runs."""
+        r = (ctypes.c_int * self._n)()
+        c = (ctypes.c_int * self._n)()
+        lib().fddm_batch_spatial_kernels(self._h, r, c)
+        return [(bool(a), bool(b)) for a, b in zip(r, c)]
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -9,10 +9,11 @@
 // host round trips between the 4 x n kernel launches.
 //
 // Autotuner: for each distinct (mode, dtype, H, W, nx, ny, precision,
-// window) the candidate frame-block sizes that fit LDS are timed on a sample
-// of the shape's longest stack (the spatial stages' HIP-event times over a
-// few executions, best of two) and the fastest is cached for the life of the
-// process; a smaller block must win by more than 3 %.
+// window) the candidate frame-block sizes that fit LDS, and for each the
+// mixed-radix or the generic form of the row and column kernels where the
+// mixed-radix ones apply, are timed on a sample of the shape's longest stack
+// (the spatial stages' HIP-event times, best of two rounds); the choice is
+// cached for the life of the process.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -50,14 +51,24 @@ struct TuneKey {
   }
 };
 
+// Tuned choice for one shape: frames per block and, where the mixed-radix
+// row / column kernels apply, whether the generic workgroup kernels run
+// instead.
+struct TuneChoice {
+  int tb = 0, rows_generic = 0, cols_generic = 0;
+};
 std::mutex g_tune_mu;
-std::map<TuneKey, int> g_tuned;  // frames per block
+std::map<TuneKey, TuneChoice> g_tuned;
 
 constexpr uint64_t kTuneFrames = 512;
 
-// Times one execution of a plan over the first `frames` frames of `it`.
-int time_candidate(const fddm_batch_item& it, uint64_t frames, int tb, hipStream_t s, float* ms, char* err,
-                   size_t errlen) {
+// Spatial-stage times of one candidate over the first `frames` frames of
+// `it`: frame-0 spectra, row pass and column pass, the best of two rounds of
+// three executions.  *forms receives whether the mixed-radix row / column
+// kernels ran (with rows_generic / cols_generic 0).
+int time_candidate(const fddm_batch_item& it, uint64_t frames, int tb, int rows_generic, int cols_generic,
+                   hipStream_t s, double* f0_ms, double* row_ms, double* col_ms, int* rows_mr, int* cols_mr,
+                   char* err, size_t errlen) {
   std::vector<uint32_t> lags;
   for (uint64_t d = 1; d < frames && lags.size() < 64; ++d) lags.push_back(static_cast<uint32_t>(d));
   uint64_t nt = 2 * frames;
@@ -65,6 +76,8 @@ int time_candidate(const fddm_batch_item& it, uint64_t frames, int tb, hipStream
   int rc = fddm::plan_create_ex(&p, it.mode, it.dtype, frames, it.height, it.width, lags.data(), lags.size(), it.nx,
                                 it.ny, nt, it.window, it.window_f32, it.precision, it.out_dtype, tb, err, errlen);
   if (rc) return rc;
+  fddm::plan_set_generic_spatial(p, rows_generic, cols_generic);
+  fddm::plan_spatial_forms(p, rows_mr, cols_mr);
   uint64_t ws = 0, out = 0;
   fddm_plan_workspace_size(p, &ws);
   fddm_plan_output_size(p, &out);
@@ -75,52 +88,75 @@ int time_candidate(const fddm_batch_item& it, uint64_t frames, int tb, hipStream
   }
   void* d_ws = buf;
   void* d_out = static_cast<unsigned char*>(buf) + align256(ws);
-  // the spatial stages only (frame-0 spectra, rows, columns): the frame
-  // block size changes nothing else
   if (!rc) rc = fddm_plan_execute(p, it.d_img, d_out, d_ws, s, err, errlen);  // warm-up
-  if (!rc) rc = fddm_plan_timing_begin(p, 3, err, errlen);
-  for (int i = 0; i < 3 && !rc; ++i) rc = fddm_plan_execute(p, it.d_img, d_out, d_ws, s, err, errlen);
-  double stage[5] = {0, 0, 0, 0, 0};
-  int steps = 0;
-  if (!rc) rc = fddm_plan_timing_end(p, stage, &steps, err, errlen);
-  if (!rc) *ms = static_cast<float>((stage[0] + stage[1] + stage[2]) / std::max(steps, 1));
+  *f0_ms = *row_ms = *col_ms = 1e30;
+  for (int round = 0; round < 2 && !rc; ++round) {
+    if (!rc) rc = fddm_plan_timing_begin(p, 3, err, errlen);
+    for (int i = 0; i < 3 && !rc; ++i) rc = fddm_plan_execute(p, it.d_img, d_out, d_ws, s, err, errlen);
+    double stage[5] = {0, 0, 0, 0, 0};
+    int steps = 0;
+    if (!rc) rc = fddm_plan_timing_end(p, stage, &steps, err, errlen);
+    if (!rc) {
+      const double n = std::max(steps, 1);
+      *f0_ms = std::min(*f0_ms, stage[0] / n);
+      *row_ms = std::min(*row_ms, stage[1] / n);
+      *col_ms = std::min(*col_ms, stage[2] / n);
+    }
+  }
   (void)hipStreamSynchronize(s);
   (void)hipFree(buf);
   fddm_plan_destroy(p);
   return rc;
 }
 
-int tuned_frames_per_block(const fddm_batch_item& it, hipStream_t s, int* tb_out, char* err, size_t errlen) {
+// The choice for the shape of it: for each frame-block size (16, 8, 4) the row and
+// the column kernel form are chosen apart (the generic form must win by more
+// than 3 %), then the block size with the shortest spatial stage (a smaller
+// block must win by more than 3 %, so noise does not pick more, shorter runs).
+int tuned_choice(const fddm_batch_item& it, hipStream_t s, TuneChoice* out, char* err, size_t errlen) {
   const TuneKey k{it.mode, it.dtype, it.precision, it.out_dtype, it.window != nullptr, it.height, it.width, it.nx,
                   it.ny};
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto f = g_tuned.find(k);
     if (f != g_tuned.end()) {
-      *tb_out = f->second;
+      *out = f->second;
       return FDDM_OK;
     }
   }
   const uint64_t frames = std::min<uint64_t>(it.length, kTuneFrames);
-  // largest block first; a smaller one must be faster by more than 3 % (the
-  // best of two timings each), so noise does not pick more, shorter runs
-  int best = 0;
-  float best_ms = 0;
+  TuneChoice best;
+  double best_ms = 0;
   for (int tb : {16, 8, 4}) {
-    float ms = 0, ms2 = 0;
+    double f0 = 0, rows = 0, cols = 0, f0g = 0, rowsg = 0, colsg = 0;
+    int rows_mr = 0, cols_mr = 0, r2 = 0, c2 = 0;
     char e2[256];
-    if (time_candidate(it, frames, tb, s, &ms, e2, sizeof e2) != FDDM_OK) continue;  // does not fit: skip
-    if (time_candidate(it, frames, tb, s, &ms2, e2, sizeof e2) == FDDM_OK) ms = std::min(ms, ms2);
-    if (!best || ms < 0.97f * best_ms) {
-      best = tb;
+    if (time_candidate(it, frames, tb, 0, 0, s, &f0, &rows, &cols, &rows_mr, &cols_mr, e2, sizeof e2) != FDDM_OK)
+      continue;  // does not fit: skip
+    TuneChoice c;
+    c.tb = tb;
+    if ((rows_mr || cols_mr) &&
+        time_candidate(it, frames, tb, 1, 1, s, &f0g, &rowsg, &colsg, &r2, &c2, e2, sizeof e2) == FDDM_OK) {
+      if (rows_mr && rowsg < 0.97 * rows) {
+        c.rows_generic = 1;
+        rows = rowsg;
+      }
+      if (cols_mr && colsg < 0.97 * cols) {
+        c.cols_generic = 1;
+        cols = colsg;
+      }
+    }
+    const double ms = f0 + rows + cols;
+    if (!best.tb || ms < 0.97 * best_ms) {
+      best = c;
       best_ms = ms;
     }
   }
-  if (!best) return set_err(err, errlen, FDDM_ERR_SIZE, "no frame-block size fits %llux%llu",
-                            (unsigned long long)it.height, (unsigned long long)it.width);
+  if (!best.tb) return set_err(err, errlen, FDDM_ERR_SIZE, "no frame-block size fits %llux%llu",
+                               (unsigned long long)it.height, (unsigned long long)it.width);
   std::lock_guard<std::mutex> g(g_tune_mu);
   g_tuned[k] = best;
-  *tb_out = best;
+  *out = best;
   return FDDM_OK;
 }
 
@@ -186,11 +222,11 @@ int fddm_batch_create(fddm_batch** out, const fddm_batch_item* items, int n, int
   }
   for (int i = 0; i < n; ++i) {
     const fddm_batch_item& it = items[i];
-    int tb = 0;
+    TuneChoice choice;
     if (autotune) {
       const TuneKey k{it.mode, it.dtype, it.precision, it.out_dtype, it.window != nullptr, it.height, it.width, it.nx,
                       it.ny};
-      const int rc = tuned_frames_per_block(items[longest[k]], tune_stream, &tb, err, errlen);
+      const int rc = tuned_choice(items[longest[k]], tune_stream, &choice, err, errlen);
       if (rc) {
         (void)hipStreamDestroy(tune_stream);
         return fail(rc);
@@ -199,11 +235,12 @@ int fddm_batch_create(fddm_batch** out, const fddm_batch_item* items, int n, int
     fddm_plan* p = nullptr;
     const int rc = fddm::plan_create_ex(&p, it.mode, it.dtype, it.length, it.height, it.width, it.lags,
                                         it.num_lags, it.nx, it.ny, it.nt, it.window, it.window_f32, it.precision,
-                                        it.out_dtype, tb, err, errlen);
+                                        it.out_dtype, choice.tb, err, errlen);
     if (rc) {
       (void)hipStreamDestroy(tune_stream);
       return fail(rc);
     }
+    fddm::plan_set_generic_spatial(p, choice.rows_generic, choice.cols_generic);
     b->plans.push_back(p);
     uint64_t ws = 0;
     fddm_plan_workspace_size(p, &ws);
@@ -284,6 +321,12 @@ int fddm_batch_launch_eager(fddm_batch* b, void* stream, char* err, size_t errle
 int fddm_batch_frames_per_block(const fddm_batch* b, int* tb) {
   if (!b || !tb) return FDDM_ERR_ARG;
   for (size_t i = 0; i < b->plans.size(); ++i) tb[i] = fddm::plan_frames_per_block(b->plans[i]);
+  return FDDM_OK;
+}
+
+int fddm_batch_spatial_kernels(const fddm_batch* b, int* rows_mr, int* cols_mr) {
+  if (!b || !rows_mr || !cols_mr) return FDDM_ERR_ARG;
+  for (size_t i = 0; i < b->plans.size(); ++i) fddm::plan_spatial_forms(b->plans[i], &rows_mr[i], &cols_mr[i]);
   return FDDM_OK;
 }
 

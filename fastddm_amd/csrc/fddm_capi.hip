@@ -36,6 +36,9 @@ struct fddm_plan {
   uint64_t long_slot = 0, bytes_long = 0;
   int logn_x = 0;  // > 0: power-of-two row kernel
   int logn_y = 0;  // > 0: power-of-two column kernel
+  // 1: the generic workgroup row / column kernel even where the mixed-radix
+  // one applies (set by the batch autotuner)
+  int rows_generic = 0, cols_generic = 0;
   uint64_t Lp = 0, nblocks = 0;
   bool subtract0 = false;
   FftDesc dx{}, dy{}, dt{};
@@ -472,6 +475,7 @@ int run_spatial(fddm_plan* p, const void* d_img, void* ws, hipStream_t s, char* 
   ra.dx = p->dx;
   ra.twx = p->d_twx;
   ra.R = R;
+  ra.generic = p->rows_generic;
   ColArgs ca{};
   ca.R = R;
   ca.H = static_cast<int>(p->H);
@@ -482,6 +486,7 @@ int run_spatial(fddm_plan* p, const void* d_img, void* ws, hipStream_t s, char* 
   ca.logn = p->logn_y;
   ca.dy = p->dy;
   ca.twy = p->d_twy;
+  ca.generic = p->cols_generic;
   int rc;
   if ((rc = record(p, 0, s, err, errlen))) return rc;
   if (p->subtract0) {
@@ -736,6 +741,7 @@ RowArgs row_args(const fddm_plan* p) {
   ra.norm = 1.0 / std::sqrt(static_cast<double>(p->nx) * static_cast<double>(p->ny));
   ra.dx = p->dx;
   ra.twx = p->d_twx;
+  ra.generic = p->rows_generic;
   return ra;
 }
 
@@ -749,6 +755,7 @@ ColArgs col_args(const fddm_plan* p) {
   ca.logn = p->logn_y;
   ca.dy = p->dy;
   ca.twy = p->d_twy;
+  ca.generic = p->cols_generic;
   return ca;
 }
 
@@ -802,6 +809,18 @@ int fddm::plan_spatial_frames(fddm_plan* p, const void* img, const void* img0, u
   auto* base = static_cast<unsigned char*>(ws);
   return spatial_frames(p, img, img0, frames, p->subtract0 ? 1 : 0, base + t0 / p->Tb * rblock,
                         base + p->bytes_R + t0 * csz, p->Lp, s, err, errlen);
+}
+
+void fddm::plan_spatial_forms(const fddm_plan* p, int* rows_mr, int* cols_mr) {
+  const RowArgs ra = row_args(p);
+  const ColArgs ca = col_args(p);
+  *rows_mr = ra.logn == 0 && !ra.generic && rows_mr_ok(ra, p->precision);
+  *cols_mr = ca.logn == 0 && !ca.generic && cols_mr_ok(ca, p->precision);
+}
+
+void fddm::plan_set_generic_spatial(fddm_plan* p, int rows_generic, int cols_generic) {
+  p->rows_generic = rows_generic ? 1 : 0;
+  p->cols_generic = cols_generic ? 1 : 0;
 }
 
 PlanInfo fddm::plan_info(const fddm_plan* p) {

@@ -342,7 +342,7 @@ int rc_bucket(long long work) {
 }  // namespace
 
 hipError_t launch_rows(const RowArgs& a, int precision, hipStream_t s) {
-  if (rows_mr_ok(a, precision)) return launch_rows_mr(a, precision, s);
+  if (!a.generic && rows_mr_ok(a, precision)) return launch_rows_mr(a, precision, s);
   const uint64_t nblocks = (a.frames + a.Tb - 1) / a.Tb;
   const unsigned grid = static_cast<unsigned>(nblocks * a.H);
   const int e = rc_bucket(static_cast<long long>(a.G) * a.nx);
@@ -353,7 +353,7 @@ hipError_t launch_rows(const RowArgs& a, int precision, hipStream_t s) {
 }
 
 hipError_t launch_cols(const ColArgs& a, int precision, hipStream_t s) {
-  if (cols_mr_ok(a, precision)) return launch_cols_mr(a, precision, s);
+  if (!a.generic && cols_mr_ok(a, precision)) return launch_cols_mr(a, precision, s);
   const unsigned grid = static_cast<unsigned>(a.nblocks * a.nxh);
   const int e = rc_bucket(static_cast<long long>(a.G) * a.ny);
   if (precision == PREC_FP64) {

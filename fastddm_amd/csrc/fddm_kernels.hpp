@@ -28,6 +28,7 @@ struct RowArgs {
   FftDesc dx;
   const void* twx;
   void* R;
+  int generic;  // 1: the generic workgroup kernel even where k_rows_mr applies
 };
 
 // Stage (a) column pass + q-major store: R -> S[q][t], q = ky * nxh + kx,
@@ -41,6 +42,7 @@ struct ColArgs {
   FftDesc dy;
   const void* twy;
   void* S;
+  int generic;  // 1: the generic workgroup kernel even where k_cols_mr applies
 };
 
 // Stage (c)/(d): per-q time series -> D(q, lags), PS, VAR in the y-shifted

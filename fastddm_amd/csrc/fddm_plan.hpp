@@ -17,6 +17,11 @@ int plan_create_ex(fddm_plan** out, int mode, int dtype, uint64_t L, uint64_t H,
                    int precision, int out_dtype, int frames_per_block, char* err, size_t errlen);
 // Frames per block chosen for a plan.
 int plan_frames_per_block(const fddm_plan* p);
+// Spatial kernel forms: whether the mixed-radix row / column kernels run for
+// a plan (else the power-of-two or generic ones), and a switch to the generic
+// workgroup kernels where they would (the batch autotuner times both).
+void plan_spatial_forms(const fddm_plan* p, int* rows_mr, int* cols_mr);
+void plan_set_generic_spatial(fddm_plan* p, int rows_generic, int cols_generic);
 // fddm_plan_execute as many shorter launches (spatial stage over chunks of
 // frames_per frames, time stage over chunks of cols_per output columns; 0 =
 // no split), bit-identical to it.  Used by the request server so that large

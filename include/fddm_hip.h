@@ -288,6 +288,11 @@ int fddm_batch_launch(fddm_batch *batch, void *stream, char *err, size_t errlen)
 int fddm_batch_launch_eager(fddm_batch *batch, void *stream, char *err, size_t errlen);
 /* Frames per block of each item's plan (tb[num_items]). */
 int fddm_batch_frames_per_block(const fddm_batch *batch, int *tb);
+/* Spatial kernels of each item's plan: rows_mr[i] (cols_mr[i]) is 1 where the
+ * mixed-radix row (column) kernel runs, 0 where the power-of-two or the
+ * generic one does (with autotune, the generic form is chosen where it timed
+ * faster).  No counterpart in the reference. */
+int fddm_batch_spatial_kernels(const fddm_batch *batch, int *rows_mr, int *cols_mr);
 int fddm_batch_destroy(fddm_batch *batch);
 
 /* ---- azimuthal average of a device-resident result ----

@@ -90,3 +90,24 @@ def test_batch_rejects_bad_items(hip):
         hip.Batch([], autotune=False)
     img.free()
     out.free()
+
+
+def test_spatial_kernel_choices(hip):
+    """Without tuning every row of these non-power-of-two stacks runs on the
+    mixed-radix kernel and every column (all shorter than 256) on the generic
+    one; tuning may only switch a mixed-radix kernel to the generic form."""
+    items, _ = _items(hip)
+    default = hip.Batch(items, autotune=False)
+    tuned = hip.Batch(items, autotune=True)
+    try:
+        k0, k1 = default.spatial_kernels(), tuned.spatial_kernels()
+        assert len(k0) == len(k1) == len(STACKS)
+        assert all(rows and not cols for rows, cols in k0), k0
+        for (r0, c0), (r1, c1) in zip(k0, k1):
+            assert (not r1 or r0) and (not c1 or c0)
+    finally:
+        default.close()
+        tuned.close()
+    for it in items:
+        it["img"].free()
+        it["out"].free()
