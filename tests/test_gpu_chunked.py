@@ -22,6 +22,8 @@ CASES = [
     ("fft", "fp32", (130, 40, 40), False, "all"),
     ("diff", "fp32c", (150, 48, 48), True, 30),
     ("diff", "fp64", (90, 35, 21), False, 20),
+    # mixed-radix column kernel (ny = 300) and rows (nx = 50) in chunks
+    ("fft", "fp32c", (120, 300, 50), True, "all"),
 ]
 
 
