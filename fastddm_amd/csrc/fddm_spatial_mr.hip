@@ -63,8 +63,11 @@ __device__ __forceinline__ void wave_pass(cplx_t<T>* row, int n, int ns, const c
   // pass instead of being hoisted to the kernel entry and kept live
   asm volatile("" : "+v"(lane));
   const int m = n / R;
-  int k = lane % ns;      // k = j mod ns, advanced by 64 mod ns per item
-  const int dk = 64 % ns;
+  // k = j mod ns, advanced by 64 mod ns per item; ns is uniform, and a power
+  // of two for every pass before the first odd radix (a mask, not a division)
+  const bool p2ns = (ns & (ns - 1)) == 0;
+  int k = p2ns ? (lane & (ns - 1)) : lane % ns;
+  const int dk = p2ns ? (64 & (ns - 1)) : 64 % ns;
   C v[MB][R];
   int ob[MB];
 #pragma unroll
@@ -224,11 +227,14 @@ __global__ void __launch_bounds__(64 * NP, (mr_min_waves<KMAX, T>())) k_rows_mr(
 
     // split into the two half spectra, store R[tb][k][y][ti] (ti fastest)
     C* __restrict__ R = static_cast<C*>(a.R) + (tb * nxh * static_cast<size_t>(H) + y) * TB;
-    for (int w = tidi; w < nxh * NP; w += NT) {
-      const int pp = w & (NP - 1), k = w / NP;
+    // thread tidi takes pair row pp = tidi mod NP and k = tidi / NP + 64 i
+    // (NT = 64 NP threads): the row is fixed and the address advances
+    {
+      const int pp = tidi & (NP - 1);
       const C* const rp = buf + pp * rs;
-      sp::split_store<T>(R + static_cast<uint32_t>(k) * kstride + 2 * pp, rp[mr_pad<T>(k)],
-                         rp[mr_pad<T>(k == 0 ? 0 : nx - k)], half_norm);
+      C* dst = R + static_cast<uint32_t>(tidi / NP) * kstride + 2 * pp;
+      for (int k = tidi / NP; k < nxh; k += 64, dst += 64 * kstride)
+        sp::split_store<T>(dst, rp[mr_pad<T>(k)], rp[mr_pad<T>(k == 0 ? 0 : nx - k)], half_norm);
     }
     __syncthreads();  // the LDS rows are rewritten by the next item
   }
@@ -294,16 +300,23 @@ __global__ void __launch_bounds__(64 * TB, (mr_min_waves<KMAX, T>())) k_cols_mr(
     fft_wave_mr<T, KMAX>(col, a.dy, twl, lane);
     __syncthreads();  // every transform done
     C* __restrict__ S = static_cast<C*>(a.S) + static_cast<size_t>(kx) * a.Lp + static_cast<size_t>(tb) * TB;
-    constexpr int TH = TB / 2;
-    for (int w = tidi; w < ny * TH; w += NT) {
-      const int tp = w & (TH - 1), ky = w / TH;
-      const C z0 = buf[(2 * tp) * rs + mr_pad<T>(ky)], z1 = buf[(2 * tp + 1) * rs + mr_pad<T>(ky)];
-      C* const dst = S + static_cast<size_t>(ky) * nxh * a.Lp + 2 * tp;
-      if constexpr (sizeof(T) == 4) {
-        *reinterpret_cast<float4*>(dst) = make_float4(z0.x, z0.y, z1.x, z1.y);
-      } else {
-        dst[0] = z0;
-        dst[1] = z1;
+    // thread tidi stores frames 2 tp, 2 tp + 1 (tp = tidi mod TB/2) of the q
+    // rows ky = tidi / (TB/2) + 128 i (NT = 128 TB/2 threads)
+    constexpr int TH = TB / 2, KS = NT / TH;
+    {
+      const int tp = tidi & (TH - 1);
+      const C* const r0 = buf + (2 * tp) * rs;
+      const C* const r1 = r0 + rs;
+      const size_t qrow = static_cast<size_t>(nxh) * a.Lp;
+      C* dst = S + static_cast<size_t>(tidi / TH) * qrow + 2 * tp;
+      for (int ky = tidi / TH; ky < ny; ky += KS, dst += KS * qrow) {
+        const C z0 = r0[mr_pad<T>(ky)], z1 = r1[mr_pad<T>(ky)];
+        if constexpr (sizeof(T) == 4) {
+          *reinterpret_cast<float4*>(dst) = make_float4(z0.x, z0.y, z1.x, z1.y);
+        } else {
+          dst[0] = z0;
+          dst[1] = z1;
+        }
       }
     }
   }
