@@ -464,7 +464,7 @@ def _subset_rows(nl, L, lags):
 
 
 def large_leg(core, args):
-    """The north-star size: 4000 frames of 1024x1024 uint16 AR(1) speckle
+    """The large workload: 4000 frames of 1024x1024 uint16 AR(1) speckle
     (sigma 1.5, D 0.2, mean 1000, std 150, seed 6), lags 1..3999,
     Blackman-Harris, nt 8000, float64 output, input resident in HBM; fp32c
     and fp64 timed with per-stage HIP events, fp32c error against fp64 on a
