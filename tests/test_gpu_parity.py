@@ -140,6 +140,8 @@ def test_persistent_rows_match_oracle(hip, precision, shape, dtype, window):
     ((21, 30, 768), np.uint16, 768, 30, True),      # longest row of the mixed-radix kernels
     ((5, 768, 30), np.int16, 30, 768, False),       # longest column of the short form
     ((100, 480, 640), np.uint16, 640, 480, True),   # more items than resident workgroups
+    ((7, 392, 686), np.uint16, 686, 392, True),     # radix 7 in rows (2 7 7 7) and columns (8 7 7)
+    ((5, 270, 343), np.int16, 343, 270, False),     # odd row length 7^3; columns 2 3 3 3 5
     # long form (769 .. 2048 points)
     ((6, 40, 1280), np.uint16, 1280, 40, True),
     ((5, 1080, 30), np.uint16, 30, 1080, False),
