@@ -130,8 +130,9 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems
   }
 }
 
-// Full-width rows of 2-byte pixels (W == NX <= 512), FP32: a persistent form
-// of k_rows_p2 with one wave per pair row.  Workgroup b takes items b,
+// Full-width rows of 2-byte pixels (W == NX <= 512), FP32 and FP64 (held to
+// 64 and 128 VGPRs): a persistent form of k_rows_p2 with one wave per pair
+// row.  Workgroup b takes items b,
 // b + grid, ... (item = tb * H + y); wave p loads, converts and transforms
 // pair row p in its own LDS region with no workgroup barrier (the 32 waves
 // of the four resident workgroups per CU overlap one another's loads).  Two
@@ -143,7 +144,7 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems
 // spill reloads waited (vmcnt) for every load and store in flight
 // (0.73 ms at 512^2 x 2000 with that prefetch, 0.68-0.69 ms without).
 template <typename Tin, typename T, int LOGX, int NP>
-__global__ void __launch_bounds__(64 * NP, 8) k_rows_p2_persist(RowArgs a, uint32_t nitems) {
+__global__ void __launch_bounds__(64 * NP, (sizeof(T) == 8 ? 4 : 8)) k_rows_p2_persist(RowArgs a, uint32_t nitems) {
   using C = cplx_t<T>;
   constexpr int NT = 64 * NP;
   constexpr int NX = 1 << LOGX, NXH = NX / 2 + 1, TB = 2 * NP;
@@ -312,9 +313,10 @@ template <int LOGX, typename Tin, typename T, int NP>
 hipError_t rows_np(const RowArgs& a, unsigned grid, hipStream_t s) {
   const size_t lds_rows = sizeof(cplx_t<T>) * lds_padded_size(NP * (1 << LOGX));
   const size_t lds = lds_rows;
-  // persistent form (FP32, 2-byte pixels, full rows up to 512 wide); as many
-  // workgroups as stay resident
-  if constexpr (sizeof(Tin) == 2 && sizeof(T) == 4 && LOGX <= 9) {
+  // persistent form (2-byte pixels, full rows up to 512 wide; FP64 too: the
+  // 512^2 x 2000 fp64 row pass 1.75 -> 1.49 ms); as many workgroups as stay
+  // resident
+  if constexpr (sizeof(Tin) == 2 && LOGX <= 9) {
     if (a.W == (1 << LOGX)) {
       constexpr SpPlan P = sp_plan(LOGX);
       const size_t lds = lds_rows + sizeof(cplx_t<T>) * P.r0 * P.r1 * (1 + P.r2);  // + the LDS twiddles
