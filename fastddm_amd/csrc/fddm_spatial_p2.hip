@@ -253,12 +253,12 @@ __global__ void __launch_bounds__(NT) k_cols_p2(ColArgs a, uint32_t nitems) {
   const C* __restrict__ src = static_cast<const C*>(a.R) + (tb * nxh + kx) * static_cast<size_t>(H) * TB;
   const int nin = H * TB;
   const C* __restrict__ tw = static_cast<const C*>(a.twy);
-  if constexpr (sizeof(T) == 4 && LOGY <= 10) {
+  if constexpr (LOGY <= 10) {
     // first pass straight from the slab: butterfly j of transform ti takes
     // y = j + r NY/R0, element y TB + ti (rows y >= H are the zero padding);
     // neighbouring lanes read neighbouring elements.  (Measured faster for
-    // FP32 columns up to 1024, whose first pass is radix 8; longer columns
-    // and FP64 keep the staged copy below.)
+    // columns up to 1024, whose first pass is radix 8: FP64 1024^2 x 4000
+    // 18.2 -> 15.2 ms; longer columns keep the staged copy below.)
     constexpr int M0 = NY / P.r0;
     C v[A0::MB][P.r0];
 #pragma unroll
