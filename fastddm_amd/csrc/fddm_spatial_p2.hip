@@ -130,9 +130,9 @@ __global__ void __launch_bounds__(kThreads) k_rows_p2(RowArgs a, uint32_t nitems
   }
 }
 
-// Full-width rows of 2-byte pixels (W == NX <= 512), FP32 and FP64 (held to
-// 64 and 128 VGPRs): a persistent form of k_rows_p2 with one wave per pair
-// row.  Workgroup b takes items b,
+// Full-width rows (W == NX <= 512) of pixels of up to 4 bytes, FP32 and FP64
+// (held to 64 and 128 VGPRs): a persistent form of k_rows_p2 with one wave
+// per pair row.  Workgroup b takes items b,
 // b + grid, ... (item = tb * H + y); wave p loads, converts and transforms
 // pair row p in its own LDS region with no workgroup barrier (the 32 waves
 // of the four resident workgroups per CU overlap one another's loads).  Two
@@ -313,10 +313,11 @@ template <int LOGX, typename Tin, typename T, int NP>
 hipError_t rows_np(const RowArgs& a, unsigned grid, hipStream_t s) {
   const size_t lds_rows = sizeof(cplx_t<T>) * lds_padded_size(NP * (1 << LOGX));
   const size_t lds = lds_rows;
-  // persistent form (2-byte pixels, full rows up to 512 wide; FP64 too: the
-  // 512^2 x 2000 fp64 row pass 1.75 -> 1.49 ms); as many workgroups as stay
-  // resident
-  if constexpr (sizeof(Tin) == 2 && LOGX <= 9) {
+  // persistent form (pixels of up to 4 bytes in whole 8-byte vectors per
+  // lane, full rows up to 512 wide, FP32 and FP64: 512^2 x 2000 fp64 rows
+  // 1.75 -> 1.49 ms, uint8 fp32c 0.78 -> 0.61 ms, float32 fp64 1.07 -> 0.77
+  // ms at 1000 frames); as many workgroups as stay resident
+  if constexpr (sizeof(Tin) <= 4 && LOGX <= 9 && ((1 << LOGX) * sizeof(Tin) / 8) % 64 == 0) {
     if (a.W == (1 << LOGX)) {
       constexpr SpPlan P = sp_plan(LOGX);
       const size_t lds = lds_rows + sizeof(cplx_t<T>) * P.r0 * P.r1 * (1 + P.r2);  // + the LDS twiddles

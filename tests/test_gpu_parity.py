@@ -101,17 +101,26 @@ def test_spatial_spectra_match_oracle(hip, precision, shape, nx, ny, window):
     assert err < (1e-13 if precision == "fp64" else 2e-6), err
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+@pytest.mark.parametrize("precision", ["fp32", "fp32c", "fp64"])
 @pytest.mark.parametrize("shape,dtype,window", [
     ((131, 256, 256), np.uint16, True),   # 9 frame blocks x 256 rows; odd L: last pair half empty
     ((70, 512, 512), np.int16, False),    # 5 frame blocks x 512 rows, signed pixels
+    ((67, 400, 512), np.uint8, True),     # 8 pixels per vector
+    ((45, 300, 256), np.int32, False),    # 2 pixels per vector
+    ((38, 256, 512), np.float32, True),
+    ((21, 150, 256), np.uint32, True),
 ])
 def test_persistent_rows_match_oracle(hip, precision, shape, dtype, window):
     """More (row, frame block) items than resident workgroups, so each
-    workgroup of the persistent row kernel loops and prefetches."""
+    workgroup of the persistent row kernel loops over items; every pixel
+    size it takes."""
     L, H, W = shape
-    info = np.iinfo(dtype)
-    img = np.random.default_rng(13).integers(max(info.min, -3000), 4000, shape).astype(dtype)
+    rng = np.random.default_rng(13)
+    if np.issubdtype(dtype, np.integer):
+        info = np.iinfo(dtype)
+        img = rng.integers(max(info.min, -3000), min(info.max, 4000), shape, endpoint=True).astype(dtype)
+    else:
+        img = (rng.random(shape) * 4000).astype(dtype)
     win = np.random.default_rng(14).random((H, W)) if window else None
     plan = hip.Plan("fft", img.dtype, shape, [1], W, H, next_fast_len(2 * L, "hip"), win, precision)
     d_img = hip.DeviceBuffer.from_array(img)
@@ -122,7 +131,7 @@ def test_persistent_rows_match_oracle(hip, precision, shape, dtype, window):
         base = base - base[0]
     ref = onp.spectra(base, W, H, win).reshape(L, -1).T
     err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
-    assert err < 2e-6, err
+    assert err < (1e-13 if precision == "fp64" else 2e-6), err
     plan.destroy()
     d_img.free()
     d_ws.free()
