@@ -20,6 +20,9 @@ STACKS = [
     (33, 30, 42, "diff", "fp32c"),
     (90, 36, 60, "fft", "fp64"),
     (20, 50, 70, "diff", "fp64"),
+    # power-of-two rows of 512: the persistent row kernel, whose FP64 form
+    # with 16-frame blocks (a tuning candidate) needs more than 64 KiB of LDS
+    (34, 40, 512, "fft", "fp64"),
     # long series: the time plans that run two output columns per workgroup
     # (an even and an odd number of columns)
     (2500, 12, 10, "fft", "fp32c"),
@@ -93,16 +96,17 @@ def test_batch_rejects_bad_items(hip):
 
 
 def test_spatial_kernel_choices(hip):
-    """Without tuning every row of these non-power-of-two stacks runs on the
-    mixed-radix kernel and every column (all shorter than 256) on the generic
-    one; tuning may only switch a mixed-radix kernel to the generic form."""
+    """Without tuning every row of the non-power-of-two stacks runs on the
+    mixed-radix kernel (the 512-wide one on the power-of-two kernel) and every
+    column (all shorter than 256) on the generic one; tuning may only switch a
+    mixed-radix kernel to the generic form."""
     items, _ = _items(hip)
     default = hip.Batch(items, autotune=False)
     tuned = hip.Batch(items, autotune=True)
     try:
         k0, k1 = default.spatial_kernels(), tuned.spatial_kernels()
         assert len(k0) == len(k1) == len(STACKS)
-        assert all(rows and not cols for rows, cols in k0), k0
+        assert all(rows == (W != 512) and not cols for (rows, cols), (L, H, W, m, p) in zip(k0, STACKS)), k0
         for (r0, c0), (r1, c1) in zip(k0, k1):
             assert (not r1 or r0) and (not c1 or c0)
     finally:
