@@ -512,9 +512,22 @@ int run_spatial(fddm_plan* p, const void* d_img, void* ws, hipStream_t s, char* 
   return FDDM_OK;
 }
 
+// The kernels read pixels in 8-byte vectors and write spectra in 16-byte
+// vectors; the output is written per element.
+int check_alignment(const fddm_plan* p, const void* d_img, const void* d_out, const void* ws, char* err,
+                    size_t errlen) {
+  const uintptr_t out_elem = p->out_dtype == OUT_F32 ? 4 : 8;
+  if (reinterpret_cast<uintptr_t>(d_img) % 8 || (ws && reinterpret_cast<uintptr_t>(ws) % 16) ||
+      (d_out && reinterpret_cast<uintptr_t>(d_out) % out_elem))
+    return set_err(err, errlen, FDDM_ERR_ARG,
+                   "device buffers must be aligned: input to 8 bytes, workspace to 16, output to its element");
+  return FDDM_OK;
+}
+
 int plan_execute(fddm_plan* p, const void* d_img, void* d_out, void* ws, hipStream_t s, char* err,
                  size_t errlen) {
   if (!p || !d_img || !d_out || !ws) return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument to execute");
+  if (const int rc = check_alignment(p, d_img, d_out, ws, err, errlen)) return rc;
   int rc = run_spatial(p, d_img, ws, s, err, errlen);
   if (rc) return rc;
   auto* base = static_cast<unsigned char*>(ws);
@@ -903,6 +916,7 @@ int fddm::plan_finish_from_spectra(fddm_plan* p, const void* frame0, const void*
 int fddm::plan_execute_pieces(fddm_plan* p, const void* d_img, void* d_out, void* ws, hipStream_t s,
                               uint64_t frames_per, int cols_per, char* err, size_t errlen) {
   if (!p || !d_img || !d_out || !ws) return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument to execute");
+  if (const int rc = check_alignment(p, d_img, d_out, ws, err, errlen)) return rc;
   const int nq = static_cast<int>(p->nq);
   if ((frames_per == 0 || frames_per >= p->L) && (cols_per <= 0 || cols_per >= nq))
     return plan_execute(p, d_img, d_out, ws, s, err, errlen);
@@ -1377,6 +1391,7 @@ int fddm_plan_spectra(fddm_plan* plan, const void* d_img, void* d_workspace, dou
                       size_t errlen) {
   if (!plan || !d_img || !d_workspace || !host_spectra)
     return set_err(err, errlen, FDDM_ERR_ARG, "NULL argument");
+  if (const int rc = check_alignment(plan, d_img, nullptr, d_workspace, err, errlen)) return rc;
   const int cap = plan->timing_cap;
   plan->timing_cap = 0;
   int rc = run_spatial(plan, d_img, d_workspace, nullptr, err, errlen);

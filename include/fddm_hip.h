@@ -174,7 +174,9 @@ int fddm_plan_destroy(fddm_plan *plan);
 int fddm_plan_workspace_size(const fddm_plan *plan, uint64_t *bytes);
 /* Output bytes: (num_lags+2) * ny * (nx/2+1) * sizeof(out element). */
 int fddm_plan_output_size(const fddm_plan *plan, uint64_t *bytes);
-/* d_img, d_out, d_workspace: device pointers; stream: hipStream_t or NULL. */
+/* d_img, d_out, d_workspace: device pointers (d_img 8-byte aligned,
+ * d_workspace 16-byte aligned, d_out aligned to its element; hipMalloc gives
+ * 256 bytes); stream: hipStream_t or NULL. */
 int fddm_plan_execute(fddm_plan *plan, const void *d_img, void *d_out, void *d_workspace,
                       void *stream, char *err, size_t errlen);
 /* fddm_plan_execute without a workspace argument, for callers that cannot

@@ -379,3 +379,30 @@ def test_too_long_series_is_rejected(hip):
     img = np.zeros((40000, 4, 4), np.uint8)
     with pytest.raises(RuntimeError, match="too long"):
         hip.ddm_fft(img, [1], 4, 4, 80000, None, precision="fp64")
+
+
+def test_execute_refuses_misaligned_buffers(hip):
+    """Input, workspace and output pointers below the kernels' vector widths
+    are refused before any launch (input 8 bytes, workspace 16, output its
+    element); aligned ones run."""
+    import ctypes
+
+    L, H, W = 8, 16, 16
+    plan = hip.Plan("fft", np.uint16, (L, H, W), [1, 2], W, H, 16, None, "fp32c")
+    d_img = hip.DeviceBuffer(L * H * W * 2 + 64)
+    d_out = hip.DeviceBuffer(plan.output_size() + 64)
+    d_ws = hip.DeviceBuffer(plan.workspace_size() + 64)
+    err = ctypes.create_string_buffer(256)
+    try:
+        for img_off, ws_off, out_off in [(4, 0, 0), (2, 0, 0), (0, 8, 0), (0, 0, 4)]:
+            rc = hip.lib().fddm_plan_execute(plan.handle, ctypes.c_void_p(d_img.ptr.value + img_off),
+                                             ctypes.c_void_p(d_out.ptr.value + out_off),
+                                             ctypes.c_void_p(d_ws.ptr.value + ws_off), None, err, 256)
+            assert rc != 0 and b"aligned" in err.value, (img_off, ws_off, out_off, rc, err.value)
+        rc = hip.lib().fddm_plan_execute(plan.handle, d_img.ptr, d_out.ptr, d_ws.ptr, None, err, 256)
+        assert rc == 0, err.value
+        hip.synchronize()
+    finally:
+        plan.destroy()
+        for b in (d_img, d_out, d_ws):
+            b.free()
