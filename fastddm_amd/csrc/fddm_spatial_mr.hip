@@ -40,9 +40,10 @@ constexpr int mr_min_waves() {
 }
 
 // LDS slot of element e of a transform row.  FP32 rows are not padded:
-// those kernels are bound by VALU issue, and the pad slot every 32 elements
-// of lds_pad (fddm_fft.hpp) costs three more instructions per access than
-// the bank conflicts of the early passes' strided writes cost (row pass
+// those kernels spend most of their time issuing VALU instructions, and the
+// pad slot every 32 elements of lds_pad (fddm_fft.hpp) costs three more
+// instructions per access than the bank conflicts of the early passes'
+// strided writes cost (row pass
 // 7.72 -> 7.21 ms, columns 8.76 -> 8.63 ms over the 48-stack batch's
 // shapes).  FP64 rows (16-byte elements) keep the pad (480x640 x 1000: rows
 // 1.15 ms padded, 1.18 ms not; columns 1.40 and 1.48 ms).
