@@ -12,6 +12,9 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <deque>
 #include <mutex>
 #include <new>
 #include <thread>
@@ -595,28 +598,105 @@ struct HostPathCache {
   hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
+// Copy workers started once per process and kept until it ends.  A call of
+// parallel_memcpy queues one job of `parts` slices; the calling thread copies
+// slices too, so a job finishes even when every worker is busy with other
+// callers' jobs.  Creating and joining threads per call cost about 40 us per
+// thread, paid on every 64 MiB staging chunk.
+class CopyPool {
+ public:
+  static CopyPool& get() {
+    static CopyPool* pool = new CopyPool();  // never destroyed: its workers live until the process ends
+    return *pool;
+  }
+  unsigned workers() const { return nworkers_; }
+  void run(char* dst, const char* src, size_t n, unsigned parts) {
+    Job job;
+    job.dst = dst;
+    job.src = src;
+    job.n = n;
+    job.parts = parts;
+    job.slice = (n + parts - 1) / parts;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      jobs_.push_back(&job);
+    }
+    cv_.notify_all();
+    work(job);
+    {
+      std::lock_guard<std::mutex> g(mu_);  // no worker joins the job after this
+      auto it = std::find(jobs_.begin(), jobs_.end(), &job);
+      if (it != jobs_.end()) jobs_.erase(it);
+    }
+    std::unique_lock<std::mutex> lk(job.mu);
+    job.cv.wait(lk, [&] { return job.done == job.parts && job.users == 0; });
+  }
+
+ private:
+  struct Job {
+    char* dst;
+    const char* src;
+    size_t n, slice;
+    unsigned parts;
+    std::atomic<unsigned> next{0};  // next unclaimed slice
+    unsigned done = 0, users = 0;   // slices copied, workers inside work(); guarded by mu
+    std::mutex mu;
+    std::condition_variable cv;
+  };
+  static void work(Job& j) {
+    for (;;) {
+      const unsigned i = j.next.fetch_add(1);
+      if (i >= j.parts) return;
+      const size_t b = i * j.slice, e = std::min(j.n, b + j.slice);
+      if (b < e) std::memcpy(j.dst + b, j.src + b, e - b);
+      std::lock_guard<std::mutex> g(j.mu);
+      if (++j.done == j.parts) j.cv.notify_all();
+    }
+  }
+  void worker() {
+    for (;;) {
+      Job* j = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !jobs_.empty(); });
+        j = jobs_.front();
+        if (j->next.load() >= j->parts) {  // every slice claimed: the job leaves the queue
+          jobs_.pop_front();
+          continue;
+        }
+        std::lock_guard<std::mutex> g(j->mu);
+        ++j->users;
+      }
+      work(*j);
+      std::lock_guard<std::mutex> g(j->mu);
+      --j->users;
+      j->cv.notify_all();
+    }
+  }
+  CopyPool() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    nworkers_ = std::min(15u, hw > 1 ? hw - 1 : 1u);
+    for (unsigned i = 0; i < nworkers_; ++i) std::thread([this] { worker(); }).detach();
+  }
+  unsigned nworkers_ = 0;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<Job*> jobs_;
+};
+
 }  // namespace
 
-// memcpy split over host threads (pageable <-> pinned staging); the caller's
-// pages are touched by several threads at once.
+// memcpy split over the copy workers (pageable <-> pinned staging); the
+// caller's pages are touched by several threads at once.
 void fddm::parallel_memcpy(void* dst, const void* src, size_t n) {
   const size_t per = size_t(4) << 20;
-  unsigned nt = static_cast<unsigned>(std::min<size_t>((n + per - 1) / per, 16));
-  const unsigned hw = std::thread::hardware_concurrency();
-  if (hw && nt > hw) nt = hw;
-  if (nt <= 1) {
+  const unsigned parts = static_cast<unsigned>(std::min<size_t>((n + per - 1) / per, 16));
+  if (parts <= 1) {
     std::memcpy(dst, src, n);
     return;
   }
-  std::vector<std::thread> th;
-  const size_t slice = (n + nt - 1) / nt;
-  for (unsigned i = 1; i < nt; ++i) {
-    const size_t b = i * slice, e = std::min(n, b + slice);
-    if (b < e)
-      th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + b, static_cast<const char*>(src) + b, e - b); });
-  }
-  std::memcpy(dst, src, std::min(n, slice));
-  for (auto& t : th) t.join();
+  CopyPool& pool = CopyPool::get();
+  pool.run(static_cast<char*>(dst), static_cast<const char*>(src), n, std::min(parts, pool.workers() + 1));
 }
 
 namespace {
