@@ -699,6 +699,21 @@ void fddm::parallel_memcpy(void* dst, const void* src, size_t n) {
   pool.run(static_cast<char*>(dst), static_cast<const char*>(src), n, std::min(parts, pool.workers() + 1));
 }
 
+// Done by a kernel where n and the addresses are multiples of 16, else by the
+// DMA engine.  The DMA engine's device-to-host rate fell from about 45 to
+// 28 GB/s for source buffers allocated where large buffers had just been freed
+// (a result buffer allocated after another analysis's buffers were released);
+// the kernel copies ran at 45 GB/s either way.
+hipError_t fddm::copy_to_pinned(void* pin, const void* d_src, size_t n, hipStream_t s) {
+  void* dp = nullptr;
+  if (n % 16 == 0 && reinterpret_cast<uintptr_t>(d_src) % 16 == 0) {
+    if (hipHostGetDevicePointer(&dp, pin, 0) == hipSuccess && dp && reinterpret_cast<uintptr_t>(dp) % 16 == 0)
+      return launch_copy_out(dp, d_src, n, s);
+    (void)hipGetLastError();  // not a device-addressable buffer: the DMA engine copies it
+  }
+  return hipMemcpyAsync(pin, d_src, n, hipMemcpyDeviceToHost, s);
+}
+
 namespace {
 
 // Host -> device through the staging pair: host threads fill one buffer while
@@ -722,8 +737,7 @@ int d2h_staged(HostPathCache& c, void* h_dst, const void* d_src, size_t n, char*
   const size_t nchunk = (n + kStageBytes - 1) / kStageBytes;
   auto issue = [&](size_t i) -> int {
     const size_t off = i * kStageBytes, m = std::min(kStageBytes, n - off);
-    FDDM_HIP(hipMemcpyAsync(c.pin[i & 1], static_cast<const char*>(d_src) + off, m, hipMemcpyDeviceToHost,
-                            c.stream));
+    FDDM_HIP(copy_to_pinned(c.pin[i & 1], static_cast<const char*>(d_src) + off, m, c.stream));
     FDDM_HIP(hipEventRecord(c.ev[i & 1], c.stream));
     return FDDM_OK;
   };

@@ -317,7 +317,7 @@ int fddm_ingest_finish(fddm_ingest* h, void* out, char* err, size_t errlen) {
     }
     if (i < nchunk) {
       const uint64_t off = i * stage, m = std::min(stage, n - off);
-      FDDM_HIP(hipMemcpyAsync(h->pin[i & 1], src + off, m, hipMemcpyDeviceToHost, h->comp));
+      FDDM_HIP(copy_to_pinned(h->pin[i & 1], src + off, m, h->comp));
       FDDM_HIP(hipEventRecord(h->copied[i & 1], h->comp));
     }
   }

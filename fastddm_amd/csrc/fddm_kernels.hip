@@ -429,4 +429,36 @@ hipError_t launch_fft_test(void* data, int n, int batch, const FftDesc& d, const
   }
 }
 
+namespace {
+
+// Device buffer -> pinned host buffer, 16 bytes per lane and four in flight
+// per lane: the GPU writes the host pages itself over the host link.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy_out(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
+                                                  uint64_t n16) {
+  const uint64_t stride = uint64_t(gridDim.x) * 256;
+  uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    const u32x4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
+  }
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+}  // namespace
+
+hipError_t launch_copy_out(void* dst, const void* src, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (n % 16 || reinterpret_cast<uintptr_t>(src) % 16 || reinterpret_cast<uintptr_t>(dst) % 16)
+    return hipErrorInvalidValue;
+  const uint64_t n16 = n / 16;
+  const unsigned blocks = static_cast<unsigned>(std::min<uint64_t>((n16 + 255) / 256, 1024));
+  hipLaunchKernelGGL(k_copy_out, dim3(blocks), dim3(256), 0, s, static_cast<const u32x4*>(src),
+                     static_cast<u32x4*>(dst), n16);
+  return hipGetLastError();
+}
+
 }  // namespace fddm

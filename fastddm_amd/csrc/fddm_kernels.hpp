@@ -109,6 +109,10 @@ hipError_t launch_cols_mr(const ColArgs& a, int precision, hipStream_t s);
 hipError_t launch_fft_test(void* data, int n, int batch, const FftDesc& d, const void* tw,
                            int precision, hipStream_t s);
 
+// n bytes from device memory to pinned host memory addressed by the device
+// (hipHostGetDevicePointer), by a kernel; n and both addresses multiples of 16.
+hipError_t launch_copy_out(void* dst, const void* src, uint64_t n, hipStream_t s);
+
 // FP64 edge correction of the compensated path (fddm_edge.hip): frames
 // [0, E) and [L - E, L) of an (L, frame_bytes) stack copied to dst, and the
 // lags lags[i0 .. nl) (n = L - dt <= E pairs) recomputed by direct sums from

@@ -51,6 +51,8 @@ int plan_finish_from_spectra(fddm_plan* p, const void* frame0, const void* edge_
 // drains pinned staging buffers at host-memory speed, and first-touches a
 // fresh destination from several threads.
 void parallel_memcpy(void* dst, const void* src, size_t n);
+// Device -> pinned host (a hipHostMalloc buffer), asynchronous on s.
+hipError_t copy_to_pinned(void* pin, const void* d_src, size_t n, hipStream_t s);
 // Frees the device buffers the streamed ingest keeps for device dev (unless
 // an ingest is using them); its pinned chunk buffers stay.
 void ingest_release_pool(int dev);
