@@ -66,6 +66,12 @@ size_t pixel_bytes(int dtype) {
 
 constexpr uint64_t kAlign = 256;
 constexpr uint64_t kPieceBytes = uint64_t(256) << 20;  // spectra per launch of a large request
+// Host inputs of at least kStageMin bytes cross the host link through two
+// pinned buffers of kStageBytes per slot (allocated at the slot's first such
+// request): host threads fill one while the DMA engine drains the other.
+// Smaller inputs are copied by the runtime from the caller's pages directly.
+constexpr uint64_t kStageBytes = uint64_t(32) << 20;
+constexpr uint64_t kStageMin = uint64_t(8) << 20;
 uint64_t align_up(uint64_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
 
 // First-fit allocator over one device range; blocks are kept sorted by
@@ -167,6 +173,11 @@ struct fddm_server {
   std::mutex plan_mu;
   std::vector<std::thread> workers;
   std::vector<hipStream_t> streams;
+  struct Stage {
+    void* pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+  };
+  std::vector<Stage> stages;  // per slot; touched only by the slot's worker until destroy
   uint64_t next_id = 1;
   bool stop = false;
   int running = 0;
@@ -224,6 +235,43 @@ struct fddm_server {
     return FDDM_OK;
   }
 
+  // Host input -> device on stream st (asynchronous: the caller synchronises).
+  hipError_t copy_in(int slot, void* d_dst, const void* h_src, uint64_t n, hipStream_t st) {
+    Stage& g = stages[slot];
+    if (n >= kStageMin && !g.pin[0]) {
+      for (int k = 0; k < 2; ++k) {
+        if (hipHostMalloc(&g.pin[k], kStageBytes, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&g.ev[k], hipEventDisableTiming) != hipSuccess) {
+          (void)hipGetLastError();
+          free_stage(g);  // pinned memory short: the runtime copies this input itself
+          break;
+        }
+      }
+    }
+    if (n < kStageMin || !g.pin[0]) return hipMemcpyAsync(d_dst, h_src, n, hipMemcpyHostToDevice, st);
+    uint64_t off = 0;
+    for (int k = 0; off < n; k ^= 1) {
+      const uint64_t m = std::min(kStageBytes, n - off);
+      hipError_t e = hipEventSynchronize(g.ev[k]);  // the buffer's previous DMA is done
+      if (e != hipSuccess) return e;
+      fddm::parallel_memcpy(g.pin[k], static_cast<const unsigned char*>(h_src) + off, m);
+      if ((e = hipMemcpyAsync(static_cast<unsigned char*>(d_dst) + off, g.pin[k], m, hipMemcpyHostToDevice, st)) !=
+              hipSuccess ||
+          (e = hipEventRecord(g.ev[k], st)) != hipSuccess)
+        return e;
+      off += m;
+    }
+    return hipSuccess;
+  }
+  static void free_stage(Stage& g) {
+    for (int k = 0; k < 2; ++k) {
+      if (g.pin[k]) (void)hipHostFree(g.pin[k]);
+      if (g.ev[k]) (void)hipEventDestroy(g.ev[k]);
+      g.pin[k] = nullptr;
+      g.ev[k] = nullptr;
+    }
+  }
+
   void worker(int slot) {
     (void)hipSetDevice(device);  // validated by fddm_server_create
     hipStream_t s = streams[slot];
@@ -242,7 +290,7 @@ struct fddm_server {
       int rc = FDDM_OK;
       const void* img = r->img;
       if (!r->img_on_device) {
-        if (hipMemcpyAsync(base + r->in_off, r->img, r->in_bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+        if (copy_in(slot, base + r->in_off, r->img, r->in_bytes, s) != hipSuccess)
           rc = set_err(err, sizeof err, FDDM_ERR_HIP, "input copy failed");
         img = base + r->in_off;
       }
@@ -312,6 +360,7 @@ int fddm_server_create(fddm_server** out, int max_slots, uint64_t hbm_bytes, cha
   }
   s->arena.reset(hbm_bytes);
   s->streams.resize(max_slots);
+  s->stages.resize(max_slots);
   for (int i = 0; i < max_slots; ++i) {
     if (hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking) != hipSuccess) {
       for (int j = 0; j < i; ++j) (void)hipStreamDestroy(s->streams[j]);
@@ -352,6 +401,7 @@ int fddm_server_destroy(fddm_server* s) {
   (void)hipSetDevice(s->device);
   for (auto& kv : s->plans) fddm_plan_destroy(kv.second);
   for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+  for (auto& g : s->stages) fddm_server::free_stage(g);
   (void)hipFree(s->base);
   delete s;
   return FDDM_OK;

@@ -220,7 +220,9 @@ int fddm_stream_synchronize(void *stream, char *err, size_t errlen);
  * output [+ a device copy of a host input]) fits in the arena; admitted
  * requests run concurrently.  Results stay in the arena until released.
  * img_on_device: img is a device pointer that stays valid until the request
- * finishes; otherwise a host pointer read when the request is admitted.
+ * finishes; otherwise a host pointer read when the request is admitted
+ * (inputs of 8 MiB and more through two 32 MiB pinned buffers per slot,
+ * allocated at the slot's first such request and freed by destroy).
  * States: 0 pending, 1 running, 2 done, 3 failed.  times[3] = seconds since
  * server start at submit, admission and completion. */
 typedef struct fddm_server fddm_server;

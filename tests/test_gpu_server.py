@@ -6,6 +6,7 @@ so "the same" is bit-for-bit."""
 import numpy as np
 import pytest
 
+from fastddm_amd import next_fast_len
 from fastddm_amd.speckle import ar1_speckle
 from oracle import oracle_np as onp
 
@@ -85,6 +86,38 @@ def test_device_resident_input_matches_host_input(hip):
     finally:
         srv.close()
         d.free()
+
+
+def test_staged_host_inputs_match_device_inputs(hip):
+    """Host inputs of 8 MiB and more cross through the slot's two pinned
+    buffers in 32 MiB pieces (here two pieces, the second partial, then one
+    partial piece of an odd-sized uint8 stack on the same slot); each result
+    equals the one computed from the same frames already on the device."""
+    stacks = [ar1_speckle(700, 160, 192, seed=11),
+              ar1_speckle(401, 131, 197, mean=120.0, std=25.0, dtype=np.uint8, seed=12, clip=(0, 255))]
+    srv = hip.Server(max_slots=1, hbm_bytes=2048 * MiB)
+    bufs = []
+    try:
+        for img in stacks:
+            L, H, W = img.shape
+            print(f"staged host input {img.shape} {img.dtype} {img.nbytes} bytes", flush=True)
+            lags = np.arange(1, L)
+            nx, ny, nt = next_fast_len(W, "hip"), next_fast_len(H, "hip"), next_fast_len(2 * L, "hip")
+            d = hip.DeviceBuffer.from_array(img)
+            bufs.append(d)
+            a = srv.submit("fft", img, lags, nx, ny, nt, precision="fp32c")
+            b = srv.submit("fft", d, lags, nx, ny, nt, precision="fp32c", img_on_device=True, shape=img.shape,
+                           dtype=img.dtype)
+            srv.wait(a, 60.0)
+            srv.wait(b, 60.0)
+            assert srv.status(a)[0] == srv.status(b)[0]
+            assert np.array_equal(srv.fetch(a), srv.fetch(b))
+            srv.release(a)
+            srv.release(b)
+    finally:
+        srv.close()
+        for d in bufs:
+            d.free()
 
 
 def test_admission_follows_hbm_footprint(hip):
