@@ -904,6 +904,26 @@ int time_launch(const fddm_plan* p, const TimeArgs& ta, void* lng, const void* e
 
 }  // namespace
 
+// Device -> pageable host memory.  From 8 MiB up the bytes cross through the
+// device's staging pair (the host-buffer calls' pinned buffers and stream,
+// whose lock is held for the copy: a host-buffer call on the same device
+// waits); smaller copies are left to the runtime.  The device must be current.
+int fddm::copy_to_host(int dev, void* h_dst, const void* d_src, size_t n, char* err, size_t errlen) {
+  if (dev < 0 || dev >= 64) return set_err(err, errlen, FDDM_ERR_DEVICE, "device id %d not supported", dev);
+  if (n < (size_t(8) << 20)) {
+    FDDM_HIP(hipMemcpy(h_dst, d_src, n, hipMemcpyDeviceToHost));
+    return FDDM_OK;
+  }
+  std::lock_guard<std::mutex> lk(g_cache_mu[dev]);
+  HostPathCache& c = g_cache[dev];
+  if (!c.stream) FDDM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  for (int k = 0; k < 2; ++k) {
+    if (!c.pin[k]) FDDM_HIP(hipHostMalloc(&c.pin[k], kStageBytes, hipHostMallocDefault));
+    if (!c.ev[k]) FDDM_HIP(hipEventCreateWithFlags(&c.ev[k], hipEventDisableTiming));
+  }
+  return d2h_staged(c, h_dst, d_src, n, err, errlen);  // returns once the last piece is in h_dst
+}
+
 // plan_execute in smaller launches: the spatial stage over chunks of
 // frames_per frames (whole frame blocks) into the same workspace, the time
 // stage and its corrections over chunks of cols_per output columns.  Each

@@ -53,6 +53,9 @@ int plan_finish_from_spectra(fddm_plan* p, const void* frame0, const void* edge_
 void parallel_memcpy(void* dst, const void* src, size_t n);
 // Device -> pinned host (a hipHostMalloc buffer), asynchronous on s.
 hipError_t copy_to_pinned(void* pin, const void* d_src, size_t n, hipStream_t s);
+// Device -> pageable host, synchronous, through the device's pinned staging
+// pair from 8 MiB up (device dev current).
+int copy_to_host(int dev, void* h_dst, const void* d_src, size_t n, char* err, size_t errlen);
 // Frees the device buffers the streamed ingest keeps for device dev (unless
 // an ingest is using them); its pinned chunk buffers stay.
 void ingest_release_pool(int dev);

@@ -545,9 +545,8 @@ int fddm_server_fetch(fddm_server* s, uint64_t id, void* host_out, char* err, si
   uint64_t bytes = 0;
   int rc = fddm_server_result(s, id, &d, &bytes, err, errlen);
   if (rc) return rc;
-  if (hipSetDevice(s->device) != hipSuccess || hipMemcpy(host_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
-    return set_err(err, errlen, FDDM_ERR_HIP, "result copy failed");
-  return FDDM_OK;
+  if (hipSetDevice(s->device) != hipSuccess) return set_err(err, errlen, FDDM_ERR_HIP, "result copy failed");
+  return fddm::copy_to_host(s->device, host_out, d, bytes, err, errlen);
 }
 
 int fddm_server_release(fddm_server* s, uint64_t id) {

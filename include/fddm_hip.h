@@ -251,6 +251,9 @@ int fddm_server_next_done(fddm_server *server, double timeout_s, uint64_t *id, c
 /* Device pointer and size of a finished result (valid until release). */
 int fddm_server_result(fddm_server *server, uint64_t id, void **d_out, uint64_t *bytes, char *err,
                        size_t errlen);
+/* Copies a finished result to host memory; from 8 MiB up through the
+ * device's pinned staging pair shared with the host-buffer calls (a
+ * host-buffer call on the same device waits for it, and it for them). */
 int fddm_server_fetch(fddm_server *server, uint64_t id, void *host_out, char *err, size_t errlen);
 int fddm_server_release(fddm_server *server, uint64_t id);
 /* stats[7] = {pending, running, completed, arena bytes, arena used, arena

@@ -92,7 +92,9 @@ def test_staged_host_inputs_match_device_inputs(hip):
     """Host inputs of 8 MiB and more cross through the slot's two pinned
     buffers in 32 MiB pieces (here two pieces, the second partial, then one
     partial piece of an odd-sized uint8 stack on the same slot); each result
-    equals the one computed from the same frames already on the device."""
+    equals the one computed from the same frames already on the device, and
+    the fetched result (staged copy) equals the same plan's output copied
+    back by the runtime."""
     stacks = [ar1_speckle(700, 160, 192, seed=11),
               ar1_speckle(401, 131, 197, mean=120.0, std=25.0, dtype=np.uint8, seed=12, clip=(0, 255))]
     srv = hip.Server(max_slots=1, hbm_bytes=2048 * MiB)
@@ -111,9 +113,20 @@ def test_staged_host_inputs_match_device_inputs(hip):
             srv.wait(a, 60.0)
             srv.wait(b, 60.0)
             assert srv.status(a)[0] == srv.status(b)[0]
-            assert np.array_equal(srv.fetch(a), srv.fetch(b))
+            got = srv.fetch(a)  # results of 8 MiB and more leave through the pinned staging pair
+            assert np.array_equal(got, srv.fetch(b))
             srv.release(a)
             srv.release(b)
+            # the same plan run directly, its output copied back by the runtime
+            plan = hip.Plan("fft", img.dtype, img.shape, lags, nx, ny, nt, None, "fp32c")
+            d_out, d_ws = hip.DeviceBuffer(plan.output_size()), hip.DeviceBuffer(plan.workspace_size())
+            bufs += [d_out, d_ws]
+            plan.execute(d, d_out, d_ws)
+            hip.synchronize()
+            ref = d_out.to_array(plan.out_shape, np.float64)
+            plan.destroy()
+            assert got.nbytes >= 8 * MiB
+            assert np.array_equal(got, ref)
     finally:
         srv.close()
         for d in bufs:
