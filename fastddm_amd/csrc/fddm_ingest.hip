@@ -74,9 +74,8 @@ int set_err(char* err, size_t len, int code, const char* fmt, ...) {
 constexpr uint64_t kChunkBytes = uint64_t(64) << 20;     // input bytes per chunk (target)
 
 // Pinned chunk buffers kept per device between ingests, until the process
-// ends: host memory freed and pinned again measured slower for the result's
-// device-to-host copies (about 30 GB/s against 45 GB/s), so one ingest at a
-// time per device reuses the pool; a concurrent one pins its own.
+// ends, so an ingest pins no host memory again: one ingest at a time per
+// device reuses the pool; a concurrent one pins its own.
 struct PinPool {
   std::mutex mu;
   bool busy = false;
